@@ -1,0 +1,183 @@
+/*
+ * kwave_hip.h -- C ABI of the MI355X-native k-space first-order PSTD time loop.
+ *
+ * This library replaces the time loop of KWave.jl's
+ *   kspace_first_order(kgrid, medium, source, sensor; kwargs...) -> SimulationOutput
+ * (KWave.jl/src/solver/first_order.jl:92-281 (1D), 287-499 (2D), 505-728 (3D))
+ * and its per-step operators (KWave.jl/src/solver/first_order_steps.jl:1-825).
+ * The host wrapper (Julia `ccall`, Python `ctypes`) keeps the reference's
+ * objects and validation; everything from table generation to the last
+ * recorded sample runs behind this boundary on one GPU.
+ *
+ * Conventions
+ *   - Arrays are column-major with x fastest (Julia layout); linear indices
+ *     are 0-based and, for masks, in `findall` (column-major) order.
+ *   - Host pointers are borrowed for the duration of the call only.
+ *   - Every function returns KW_OK (0) or a negative status; the message of
+ *     the last failure on the calling thread is kw_last_error().
+ *   - One kw_sim is not reentrant; distinct sims may be driven from
+ *     different host threads.  Each sim owns one HIP stream.
+ */
+#ifndef KWAVE_HIP_H
+#define KWAVE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KW_ABI_VERSION 1
+
+enum kw_status {
+  KW_OK = 0,
+  KW_ERR_INVALID = -1,     /* bad descriptor / argument (reference: error(...)) */
+  KW_ERR_UNSUPPORTED = -2, /* valid input this build cannot run (e.g. FFT length) */
+  KW_ERR_HIP = -3,         /* HIP runtime failure */
+  KW_ERR_OOM = -4,         /* device allocation failed */
+  KW_ERR_STATE = -5        /* call out of order (e.g. sensor data before any step) */
+};
+
+/* SourceMode, KWave.jl/src/types.jl:14 (same order). */
+enum kw_source_mode { KW_DIRICHLET = 0, KW_ADDITIVE = 1, KW_ADDITIVE_NO_CORRECTION = 2 };
+
+/* alpha_mode symbols, KWave.jl/src/medium.jl:66; any other symbol acts as
+ * :stokes (first_order_steps.jl:252). */
+enum kw_alpha_mode { KW_NO_ABSORPTION = 0, KW_NO_DISPERSION = 1, KW_STOKES = 2 };
+
+/* Record fields, KWave.jl/src/sensor.jl:41-45 (bit per symbol). */
+enum kw_record {
+  KW_REC_P = 1u << 0,
+  KW_REC_P_MAX = 1u << 1,
+  KW_REC_P_MIN = 1u << 2,
+  KW_REC_P_RMS = 1u << 3,
+  KW_REC_P_FINAL = 1u << 4,
+  KW_REC_UX = 1u << 5,
+  KW_REC_UY = 1u << 6,
+  KW_REC_UZ = 1u << 7,
+  KW_REC_U_MAX = 1u << 8,
+  KW_REC_U_RMS = 1u << 9,
+  KW_REC_U_FINAL = 1u << 10,
+  KW_REC_I_AVG = 1u << 11,
+  KW_REC_I_MAX = 1u << 12
+};
+
+/* Fields readable with kw_sim_get_field. */
+enum kw_field { KW_FIELD_P = 0, KW_FIELD_UX, KW_FIELD_UY, KW_FIELD_UZ, KW_FIELD_RHOX, KW_FIELD_RHOY, KW_FIELD_RHOZ };
+
+/* Storage of the static medium maps (c0, rho0, B/A). */
+enum kw_map_storage { KW_MAPS_NATIVE = 0, KW_MAPS_FP16 = 2 };
+
+typedef struct kw_ctx kw_ctx;
+typedef struct kw_sim kw_sim;
+
+/*
+ * Simulation descriptor: everything kspace_first_order derives from
+ * (kgrid, medium, source, sensor, kwargs) before its time loop
+ * (first_order.jl:311-440 and 529-669).  Scalars that the reference keeps
+ * as Float64 stay double here; array inputs are Float64, column-major.
+ */
+typedef struct kw_sim_desc {
+  int32_t ndim;          /* 1, 2 or 3 (KWaveGrid1D/2D/3D, grid.jl:21-108) */
+  int64_t N[3];          /* Nx, Ny, Nz (unused dims = 1) */
+  double d[3];           /* dx, dy, dz [m] */
+  double dt;             /* kgrid.dt[] */
+  int64_t Nt;            /* kgrid.Nt[] (time reversal: columns of the TR data) */
+  int32_t precision;     /* 4 = Float32 kernels, 8 = Float64 kernels (data_cast) */
+  int32_t map_storage;   /* kw_map_storage */
+  int32_t pml_size[3];   /* per axis (first_order.jl:322, 540) */
+  double pml_alpha[3];
+  double c_ref;          /* mean(c0) as the host computed it (first_order.jl:361, 581) */
+
+  const double* c0;      /* sound speed map or NULL (then c0_scalar) */
+  double c0_scalar;
+  const double* rho0;    /* density map or NULL (then rho0_scalar) */
+  double rho0_scalar;
+  int32_t nonlinear;     /* medium.BonA !== nothing */
+  const double* bona;    /* B/A map or NULL (then bona_scalar) */
+  double bona_scalar;
+  int32_t absorbing;     /* !is_lossless(medium) (medium.jl:90) */
+  int32_t alpha_mode;    /* kw_alpha_mode */
+  double alpha_coeff;    /* dB/(MHz^y cm), scalar (first_order.jl:588-592) */
+  double alpha_power;    /* y */
+
+  const double* p0;      /* initial pressure (full grid) or NULL */
+  int32_t smooth_p0;     /* apply smooth(p0; restore_max=true) (filters.jl:20-60) */
+
+  /* time-varying pressure source (source.jl:24-34; first_order_steps.jl:561-715) */
+  int64_t n_psrc;
+  const int64_t* psrc_index;   /* findall(p_mask), 0-based */
+  const double* psrc_signal;   /* n_psrc x psrc_ncols, column-major */
+  int64_t psrc_ncols;
+  int32_t p_mode;              /* kw_source_mode */
+
+  /* time-varying velocity source (first_order_steps.jl:541-559, 586-617, 646-690) */
+  int64_t n_usrc;
+  const int64_t* usrc_index;   /* findall(u_mask), 0-based */
+  const double* usrc_signal[3];/* ux, uy, uz: n_usrc x usrc_ncols[d] each, or NULL */
+  int64_t usrc_ncols[3];
+  int32_t u_mode;
+
+  /* sensor (sensor.jl:31-38, 94-136) */
+  int64_t n_sensor;
+  const int64_t* sensor_index; /* findall(mask), 0-based; NULL with n_sensor>0 = Cartesian (never filled) */
+  uint32_t record;             /* kw_record bits */
+
+  /* time reversal (first_order.jl:426-433, 450-460): n_sensor x Nt data injected
+   * (Dirichlet) at sensor_index, reversed in time; NULL if not TR */
+  const double* tr_data;
+} kw_sim_desc;
+
+/* Library identity and errors. */
+int32_t kw_abi_version(void);
+const char* kw_last_error(void);
+
+/* Device context: owns nothing but the device selection. */
+int32_t kw_device_count(int32_t* count);
+int32_t kw_ctx_create(int32_t device, kw_ctx** out);
+int32_t kw_ctx_destroy(kw_ctx* ctx);
+
+/* 1 if an axis of length n runs on the line-FFT kernels, else 0. */
+int32_t kw_fft_length_supported(int64_t n);
+
+/*
+ * Build a simulation: host-side FP64 tables (k vectors, staggered shifts,
+ * kappa, absorption, PML, twiddles), device buffers, p0 initialisation
+ * (initialize_p0_*, first_order_steps.jl:58-191).  Replaces
+ * first_order.jl:146-224 / 346-440 / 564-669.
+ */
+int32_t kw_sim_create(kw_ctx* ctx, const kw_sim_desc* desc, kw_sim** out);
+
+/* Advance n_steps time steps (the loop body of first_order.jl:232-266,
+ * 448-484, 677-713); returns when they are enqueued (use kw_sim_synchronize). */
+int32_t kw_sim_run(kw_sim* sim, int64_t n_steps);
+int32_t kw_sim_synchronize(kw_sim* sim);
+int32_t kw_sim_steps_done(const kw_sim* sim, int64_t* steps);
+
+/* Copy a full field (column-major, Float64) into caller memory. */
+int32_t kw_sim_get_field(kw_sim* sim, int32_t which, double* out);
+
+/*
+ * Finalised sensor data (finalize_sensor_data!, first_order_steps.jl:818-825)
+ * as Float64 in the reference's shapes: time series n_sensor x Nt
+ * (column-major), reductions n_sensor.  In time-reversal mode KW_REC_P_FINAL
+ * returns the full field (first_order.jl:721-723).
+ */
+int32_t kw_sim_sensor_shape(kw_sim* sim, uint32_t field, int64_t* rows, int64_t* cols);
+int32_t kw_sim_get_sensor(kw_sim* sim, uint32_t field, double* out);
+
+/*
+ * Timing support for benchmarks: run n_steps and return the per-kernel-class
+ * average device time in milliseconds (HIP events on the sim's stream).
+ * names/ms have room for max_classes entries; *n_classes receives the count.
+ */
+int32_t kw_sim_profile(kw_sim* sim, int64_t n_steps, int32_t max_classes, int32_t* n_classes,
+                       const char** names, double* ms, int64_t* launches);
+
+int32_t kw_sim_destroy(kw_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KWAVE_HIP_H */

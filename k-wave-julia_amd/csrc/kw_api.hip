@@ -1,0 +1,1045 @@
+// C ABI implementation: host-side FP64 table generation, device buffers,
+// launch sequencing of one PSTD time step, sensor read-back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/kwave_hip.h"
+#include "kw_launch.hpp"
+
+using namespace kw;
+
+namespace {
+
+thread_local std::string g_err;
+
+struct KwError {
+  int32_t code;
+  std::string msg;
+};
+
+[[noreturn]] void raise(int32_t code, const std::string& m) { throw KwError{code, m}; }
+
+#define HIPC(x)                                                                         \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) raise(KW_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device memory
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t b) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = b;
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      raise(KW_ERR_OOM, "hipMalloc of " + std::to_string(b) + " bytes failed: " + hipGetErrorString(e));
+    }
+  }
+  template <typename U>
+  U* as() const {
+    return reinterpret_cast<U*>(p);
+  }
+};
+
+template <typename U>
+void upload(DevBuf& b, const std::vector<U>& h) {
+  b.alloc(h.size() * sizeof(U));
+  if (!h.empty()) HIPC(hipMemcpy(b.p, h.data(), h.size() * sizeof(U), hipMemcpyHostToDevice));
+}
+
+// ---------------------------------------------------------------------------
+// Host tables (FP64, cast once to the kernel precision)
+// ---------------------------------------------------------------------------
+// grid.jl:120-130
+std::vector<double> wavenumbers(long long n, double d) {
+  std::vector<double> k(n);
+  const double dk = 2.0 * M_PI / ((double)n * d);
+  for (long long i = 0; i < n; ++i) {
+    long long m;
+    if (n % 2 == 0)
+      m = (i < n / 2) ? i : i - n;
+    else
+      m = (i <= (n - 1) / 2) ? i : i - n;
+    k[i] = (double)m * dk;
+  }
+  return k;
+}
+
+// pml.jl:25-55
+std::vector<double> get_pml(long long n, int size, double alpha, double dt, bool staggered) {
+  std::vector<double> p(n, 1.0);
+  for (int i = 1; i <= size; ++i) {
+    const double pos = staggered ? ((double)(size - i) + 0.5) / size : (double)(size - i + 1) / size;
+    const double v = std::exp(-(alpha * std::pow(pos, 4)) * dt);
+    p[i - 1] = v;
+    p[n - i] = v;
+  }
+  return p;
+}
+
+// exp(-2 pi i m / n) with exact symmetric reduction of the angle
+void twiddles(long long n, std::vector<double>& re, std::vector<double>& im) {
+  re.resize(n);
+  im.resize(n);
+  for (long long m = 0; m < n; ++m) {
+    // angle = -2 pi m/n; evaluate via the smallest equivalent argument
+    long long mm = m;
+    double s = -1.0;
+    if (2 * mm > n) {
+      mm = n - mm;
+      s = 1.0;
+    }
+    const double a = 2.0 * M_PI * (double)mm / (double)n;
+    re[m] = std::cos(a);
+    im[m] = s * std::sin(a);
+  }
+}
+
+template <typename T>
+std::vector<Cx<T>> to_cx(const std::vector<double>& re, const std::vector<double>& im) {
+  std::vector<Cx<T>> o(re.size());
+  for (size_t i = 0; i < re.size(); ++i) o[i] = Cx<T>{(T)re[i], (T)im[i]};
+  return o;
+}
+template <typename T>
+std::vector<T> to_t(const std::vector<double>& v) {
+  std::vector<T> o(v.size());
+  for (size_t i = 0; i < v.size(); ++i) o[i] = (T)v[i];
+  return o;
+}
+
+struct AxisTables {
+  std::vector<double> twr, twi, dpr, dpi, dmr, dmi, pml, pml_sg;
+};
+
+// Staggered derivative multipliers i k exp(+-i k h/2) (grid.jl:149-153,
+// fft_utils.jl:146-175) with the inverse 1/N folded in.  The reference's
+// irfft keeps only the real part of the DC and Nyquist bins of the product;
+// taking the real part of the multiplier there reproduces that exactly for
+// the pair-of-real-lines transform.
+AxisTables axis_tables(long long n, double h, int pml_size, double pml_alpha, double dt) {
+  AxisTables A;
+  twiddles(n, A.twr, A.twi);
+  std::vector<double> k = wavenumbers(n, h);
+  A.dpr.resize(n);
+  A.dpi.resize(n);
+  A.dmr.resize(n);
+  A.dmi.resize(n);
+  for (long long i = 0; i < n; ++i) {
+    const double c = std::cos(k[i] * h / 2), s = std::sin(k[i] * h / 2);
+    // i k (c + i s) = -k s + i k c ; i k (c - i s) = k s + i k c
+    double pr = -k[i] * s, pi = k[i] * c;
+    double mr = k[i] * s, mi = k[i] * c;
+    if (i == 0 || (n % 2 == 0 && i == n / 2)) {
+      pi = 0.0;
+      mi = 0.0;
+    }
+    A.dpr[i] = pr / (double)n;
+    A.dpi[i] = pi / (double)n;
+    A.dmr[i] = mr / (double)n;
+    A.dmi[i] = mi / (double)n;
+  }
+  A.pml = get_pml(n, pml_size, pml_alpha, dt, false);
+  A.pml_sg = get_pml(n, pml_size, pml_alpha, dt, true);
+  return A;
+}
+
+// ---------------------------------------------------------------------------
+// Simulation
+// ---------------------------------------------------------------------------
+struct SimBase {
+  virtual ~SimBase() = default;
+  virtual void run(long long n) = 0;
+  virtual void sync() = 0;
+  virtual long long steps_done() const = 0;
+  virtual void get_field(int which, double* out) = 0;
+  virtual void sensor_shape(uint32_t f, long long* r, long long* c) = 0;
+  virtual void get_sensor(uint32_t f, double* out) = 0;
+  virtual void profile(long long n, int maxc, int* nc, const char** names, double* ms, long long* launches) = 0;
+};
+
+struct SrcDev {
+  DevBuf off, pos, row;
+  bool any = false;
+};
+
+struct PassCfg {
+  int n = 0;        // line length
+  int lpw = 1;      // lines (pairs) per workgroup
+  dim3 grid, block;
+  size_t shmem = 0;
+};
+
+template <typename T>
+class Sim final : public SimBase {
+ public:
+  Sim(int device, const kw_sim_desc& d) : dev_(device) { build(d); }
+  ~Sim() override {
+    if (stream_) {
+      (void)hipSetDevice(dev_);
+      (void)hipStreamSynchronize(stream_);
+      (void)hipStreamDestroy(stream_);
+    }
+  }
+
+  void run(long long n) override {
+    HIPC(hipSetDevice(dev_));
+    for (long long i = 0; i < n && t_ < Nt_; ++i) step(nullptr);
+    HIPC(hipGetLastError());
+    if (t_ == Nt_ && !finalized_) finalize();
+  }
+  void sync() override {
+    HIPC(hipSetDevice(dev_));
+    HIPC(hipStreamSynchronize(stream_));
+  }
+  long long steps_done() const override { return t_; }
+
+  void get_field(int which, double* out) override {
+    HIPC(hipSetDevice(dev_));
+    const T* src = nullptr;
+    if (which == KW_FIELD_P)
+      src = p_.as<T>();
+    else if (which >= KW_FIELD_UX && which <= KW_FIELD_UZ && which - KW_FIELD_UX < nd_)
+      src = u_[which - KW_FIELD_UX].as<T>();
+    else if (which >= KW_FIELD_RHOX && which <= KW_FIELD_RHOZ && which - KW_FIELD_RHOX < nd_)
+      src = rho_[which - KW_FIELD_RHOX].as<T>();
+    else
+      raise(KW_ERR_INVALID, "invalid field selector");
+    copy_out(src, nvox_, out);
+  }
+
+  void sensor_shape(uint32_t f, long long* r, long long* c) override {
+    if (!(rec_ & f) && !(tr_ && f == KW_REC_P_FINAL)) raise(KW_ERR_INVALID, "field not recorded");
+    *c = 1;
+    *r = nsens_;
+    if (f == KW_REC_P || f == KW_REC_UX || f == KW_REC_UY || f == KW_REC_UZ) *c = Nt_alloc_;
+    if (tr_ && f == KW_REC_P_FINAL) *r = nvox_;
+  }
+
+  void get_sensor(uint32_t f, double* out) override {
+    HIPC(hipSetDevice(dev_));
+    sync();
+    long long rows, cols;
+    sensor_shape(f, &rows, &cols);
+    if (tr_ && f == KW_REC_P_FINAL) {
+      copy_out(p_.as<T>(), nvox_, out);
+      return;
+    }
+    if (cartesian_) {
+      // Cartesian masks are preallocated and never filled (first_order.jl:417-421)
+      double init = 0.0;
+      if (f == KW_REC_P_MAX || f == KW_REC_U_MAX || f == KW_REC_I_MAX) init = -INFINITY;
+      if (f == KW_REC_P_MIN) init = INFINITY;
+      for (long long i = 0; i < rows * cols; ++i) out[i] = init;
+      return;
+    }
+    switch (f) {
+      case KW_REC_P:
+        copy_out(p_ts_.as<T>(), rows * cols, out);
+        break;
+      case KW_REC_UX:
+      case KW_REC_UY:
+      case KW_REC_UZ: {
+        const int d = f == KW_REC_UX ? 0 : (f == KW_REC_UY ? 1 : 2);
+        if (d >= nd_) {
+          for (long long i = 0; i < rows * cols; ++i) out[i] = 0.0;
+        } else {
+          copy_out(u_ts_[d].as<T>(), rows * cols, out);
+        }
+        break;
+      }
+      case KW_REC_P_MAX:
+        copy_out(p_max_.as<T>(), rows, out);
+        break;
+      case KW_REC_P_MIN:
+        copy_out(p_min_.as<T>(), rows, out);
+        break;
+      case KW_REC_P_FINAL:
+        copy_out(p_final_.as<T>(), rows, out);
+        break;
+      case KW_REC_P_RMS:
+        HIPC(hipMemcpy(out, p_rms_.p, rows * sizeof(double), hipMemcpyDeviceToHost));
+        break;
+      case KW_REC_U_MAX:
+        HIPC(hipMemcpy(out, u_max_.p, rows * sizeof(double), hipMemcpyDeviceToHost));
+        break;
+      case KW_REC_U_RMS:
+        HIPC(hipMemcpy(out, u_rms_.p, rows * sizeof(double), hipMemcpyDeviceToHost));
+        break;
+      case KW_REC_U_FINAL:
+        HIPC(hipMemcpy(out, u_final_.p, rows * sizeof(double), hipMemcpyDeviceToHost));
+        break;
+      case KW_REC_I_AVG:
+        HIPC(hipMemcpy(out, I_avg_.p, rows * sizeof(double), hipMemcpyDeviceToHost));
+        break;
+      case KW_REC_I_MAX:
+        HIPC(hipMemcpy(out, I_max_.p, rows * sizeof(double), hipMemcpyDeviceToHost));
+        break;
+      default:
+        raise(KW_ERR_INVALID, "unknown sensor field");
+    }
+  }
+
+  void profile(long long n, int maxc, int* nc, const char** names, double* ms, long long* launches) override {
+    HIPC(hipSetDevice(dev_));
+    Prof pr;
+    pr.ev.reserve(64);
+    for (long long i = 0; i < n && t_ < Nt_; ++i) step(&pr);
+    HIPC(hipStreamSynchronize(stream_));
+    std::vector<double> tot(kNumClasses, 0.0);
+    std::vector<long long> cnt(kNumClasses, 0);
+    for (auto& r : pr.ev) {
+      float e = 0.f;
+      HIPC(hipEventElapsedTime(&e, r.a, r.b));
+      tot[r.cls] += e;
+      cnt[r.cls] += 1;
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    int k = 0;
+    for (int c = 0; c < kNumClasses && k < maxc; ++c) {
+      if (cnt[c] == 0) continue;
+      names[k] = kClassNames[c];
+      ms[k] = tot[c] / (double)cnt[c];
+      launches[k] = cnt[c];
+      ++k;
+    }
+    *nc = k;
+    if (t_ == Nt_ && !finalized_) finalize();
+  }
+
+ private:
+  enum Cls { C_DIR_X, C_DIR_Y, C_DIR_Z, C_SPEC_Y_FWD, C_SPEC_Z, C_SPEC_Y_INV, C_SPEC_Y, C_C2R, C_REC, C_TR, kNumClasses };
+  static constexpr const char* kClassNames[kNumClasses] = {
+      "k_dir_pass_x_eos", "k_dir_pass_y", "k_dir_pass_z", "k_spec_pass_y_fwd", "k_spec_pass_z_mul",
+      "k_spec_pass_y_inv", "k_spec_pass_y_mul", "k_c2r_x", "k_record", "k_tr_inject"};
+  struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+  };
+  struct Prof {
+    std::vector<ProfRec> ev;
+  };
+
+  template <typename F>
+  void launch(Prof* pr, int cls, F&& f) {
+    if (!pr) {
+      f();
+      return;
+    }
+    ProfRec r;
+    r.cls = cls;
+    HIPC(hipEventCreate(&r.a));
+    HIPC(hipEventCreate(&r.b));
+    HIPC(hipEventRecord(r.a, stream_));
+    f();
+    HIPC(hipEventRecord(r.b, stream_));
+    pr->ev.push_back(r);
+  }
+
+  void copy_out(const T* dsrc, long long n, double* out) {
+    std::vector<T> h(n);
+    if (n) HIPC(hipMemcpy(h.data(), dsrc, n * sizeof(T), hipMemcpyDeviceToHost));
+    for (long long i = 0; i < n; ++i) out[i] = (double)h[i];
+  }
+
+  // ---- construction -------------------------------------------------------
+  void build(const kw_sim_desc& d) {
+    HIPC(hipSetDevice(dev_));
+    nd_ = d.ndim;
+    if (nd_ < 1 || nd_ > 3) raise(KW_ERR_INVALID, "ndim must be 1, 2 or 3");
+    for (int a = 0; a < 3; ++a) N_[a] = a < nd_ ? d.N[a] : 1;
+    for (int a = 0; a < nd_; ++a) {
+      if (N_[a] < 1) raise(KW_ERR_INVALID, "grid size must be positive");
+      if (!size_supported(N_[a]))
+        raise(KW_ERR_UNSUPPORTED, "axis length " + std::to_string(N_[a]) +
+                                      " has no line-FFT plan (supported: 8,12,16,24,32,48,64,96,128,192,256,"
+                                      "384,512,768,1024,2048,4096)");
+      if (d.pml_size[a] < 0) raise(KW_ERR_INVALID, "pml_size must be >= 0");
+      if (2LL * d.pml_size[a] > N_[a] + 0 && d.pml_size[a] > N_[a])
+        raise(KW_ERR_INVALID, "pml_size larger than the grid");
+    }
+    if (d.dt == 0.0 || d.Nt == 0)
+      raise(KW_ERR_INVALID, "Time array not set. Call make_time!(kgrid, sound_speed) before running.");
+    if (d.map_storage != KW_MAPS_NATIVE) raise(KW_ERR_UNSUPPORTED, "map storage mode not supported");
+    nvox_ = N_[0] * N_[1] * N_[2];
+    dt_ = d.dt;
+    Nt_ = d.Nt;
+    Nt_alloc_ = d.Nt;
+    tr_ = d.tr_data != nullptr;
+    HIPC(hipStreamCreate(&stream_));
+
+    const bool het_c = d.c0 != nullptr, het_r = d.rho0 != nullptr;
+    // fields
+    p_.alloc(nvox_ * sizeof(T));
+    HIPC(hipMemset(p_.p, 0, p_.bytes));
+    for (int a = 0; a < nd_; ++a) {
+      u_[a].alloc(nvox_ * sizeof(T));
+      rho_[a].alloc(nvox_ * sizeof(T));
+      HIPC(hipMemset(u_[a].p, 0, u_[a].bytes));
+      HIPC(hipMemset(rho_[a].p, 0, rho_[a].bytes));
+    }
+    // medium maps
+    if (het_r) upload(rho0_, to_t<T>(std::vector<double>(d.rho0, d.rho0 + nvox_)));
+    if (het_c) {
+      std::vector<double> c2(nvox_);
+      for (long long i = 0; i < nvox_; ++i) c2[i] = d.c0[i] * d.c0[i];
+      upload(c0sq_, to_t<T>(c2));
+    }
+    if (d.nonlinear && d.bona) upload(bona_, to_t<T>(std::vector<double>(d.bona, d.bona + nvox_)));
+
+    // axis tables
+    for (int a = 0; a < nd_; ++a) {
+      AxisTables A = axis_tables(N_[a], d.d[a], d.pml_size[a], d.pml_alpha[a], d.dt);
+      upload(tw_[a], to_cx<T>(A.twr, A.twi));
+      upload(Dp_[a], to_cx<T>(A.dpr, A.dpi));
+      upload(Dm_[a], to_cx<T>(A.dmr, A.dmi));
+      upload(pml_[a], to_t<T>(A.pml));
+      upload(pmlsg_[a], to_t<T>(A.pml_sg));
+    }
+    if (nd_ > 1) {
+      // the x half spectrum is transformed along y/z with the full-length twiddles
+    }
+    build_eos(d, het_c);
+
+    // launch geometry
+    NH_ = N_[0] / 2 + 1;
+    cfg_[0] = pass_cfg(N_[0], true, N_[1] * N_[2], 0);
+    if (nd_ > 1) cfg_[1] = pass_cfg(N_[1], false, N_[0], N_[2]);
+    if (nd_ > 2) cfg_[2] = pass_cfg(N_[2], false, N_[0], N_[1]);
+    if (nd_ > 1) spec_[0] = spec_cfg(N_[1], N_[2]);
+    if (nd_ > 2) spec_[1] = spec_cfg(N_[2], N_[1]);
+    c2r_ = pass_cfg(N_[0], true, N_[1] * N_[2], 0);
+
+    build_sources(d, het_c);
+    build_sensor(d);
+    build_args(d);
+    if (d.p0 && !tr_) init_p0(d);
+    HIPC(hipStreamSynchronize(stream_));
+  }
+
+  PassCfg pass_cfg(long long n, bool contig, long long lines, long long outer) {
+    PassCfg c;
+    c.n = (int)n;
+    const int tpl = plan_tpl((int)n);
+    const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
+    int lpw = 256 / tpl;
+    while (lpw > 1 && (size_t)lpw * line_bytes > 65536) lpw /= 2;
+    // pairs that exist
+    const long long pairs = contig ? (lines + 1) / 2 : (lines + 1) / 2;
+    while (lpw > 1 && lpw / 2 >= pairs) lpw /= 2;
+    c.lpw = lpw;
+    c.block = dim3(lpw * tpl);
+    c.shmem = (size_t)lpw * line_bytes;
+    if (c.shmem > 65536) raise(KW_ERR_UNSUPPORTED, "line of length " + std::to_string(n) + " exceeds 64 KiB of LDS");
+    if (contig) {
+      c.grid = dim3((unsigned)((pairs + lpw - 1) / lpw));
+    } else {
+      const long long xb = (pairs + lpw - 1) / lpw;
+      c.grid = dim3((unsigned)(xb * std::max<long long>(outer, 1)));
+    }
+    return c;
+  }
+  PassCfg spec_cfg(long long n, long long outer) {
+    PassCfg c;
+    c.n = (int)n;
+    const int tpl = plan_tpl((int)n);
+    const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
+    int lpw = 256 / tpl;
+    while (lpw > 1 && (size_t)lpw * line_bytes > 65536) lpw /= 2;
+    while (lpw > 1 && lpw / 2 >= NH_) lpw /= 2;
+    c.lpw = lpw;
+    c.block = dim3(lpw * tpl);
+    c.shmem = (size_t)lpw * line_bytes;
+    const long long xb = (NH_ + lpw - 1) / lpw;
+    c.grid = dim3((unsigned)(xb * std::max<long long>(outer, 1)));
+    return c;
+  }
+
+  // Equation-of-state multipliers (first_order.jl:13-42, 156-170;
+  // first_order_steps.jl:214-290) on the octant |kx| x |ky| x |kz| with the
+  // inverse-transform 1/(Nx Ny Nz) folded in.
+  void build_eos(const kw_sim_desc& d, bool het_c) {
+    const long long hx = N_[0] / 2 + 1, hy = N_[1] / 2 + 1, hz = N_[2] / 2 + 1;
+    std::vector<double> kx = wavenumbers(N_[0], d.d[0]);
+    std::vector<double> ky = nd_ > 1 ? wavenumbers(N_[1], d.d[1]) : std::vector<double>{0.0};
+    std::vector<double> kz = nd_ > 2 ? wavenumbers(N_[2], d.d[2]) : std::vector<double>{0.0};
+    const double c_ref = d.c_ref;
+    const double inv_n = 1.0 / (double)nvox_;
+    // absorption prefactors (first_order.jl:13-30, conversion.jl:34)
+    bool absorbing = d.absorbing != 0 && d.alpha_mode != KW_NO_ABSORPTION;
+    double tau = 0, eta = 0, y = d.alpha_power;
+    if (absorbing) {
+      const double a_np = d.alpha_coeff * (1e-2 / (20.0 * std::log10(std::exp(1.0)))) *
+                          std::pow(2.0 * M_PI * 1e6, -y);
+      tau = -2.0 * a_np * std::pow(c_ref, y - 1.0);
+      eta = 2.0 * a_np * std::pow(c_ref, y - 1.0) * std::tan(M_PI * y / 2.0);
+    }
+    stokes_ = absorbing && !het_c && d.alpha_mode != KW_NO_DISPERSION;
+    const double c2 = het_c ? 1.0 : d.c0_scalar * d.c0_scalar;
+    std::vector<T> m1(hx * hy * hz), m2;
+    if (stokes_) m2.resize(m1.size());
+    for (long long iz = 0; iz < hz; ++iz)
+      for (long long iy = 0; iy < hy; ++iy)
+        for (long long ix = 0; ix < hx; ++ix) {
+          const double kk = std::sqrt(kx[ix] * kx[ix] + ky[iy] * ky[iy] + kz[iz] * kz[iz]);
+          const double arg = c_ref * kk * d.dt / 2;
+          const double kap = arg == 0.0 ? 1.0 : std::sin(arg) / arg;
+          double v1 = c2 * kap, v2 = 0.0;
+          if (absorbing) {
+            const double n1 = tau * std::pow(kk, y);
+            v1 = c2 * kap * (1.0 + n1);
+            if (stokes_) v2 = -c2 * kap * (eta * std::pow(kk, y - 1.0));
+          }
+          const long long i = ix + hx * (iy + hy * iz);
+          m1[i] = (T)(v1 * inv_n);
+          if (stokes_) m2[i] = (T)(v2 * inv_n);
+        }
+    upload(M1_, m1);
+    if (stokes_) upload(M2_, m2);
+    // spectra
+    const long long lines = N_[1] * N_[2];
+    S1_.alloc((size_t)hx * lines * sizeof(Cx<T>));
+    if (stokes_) {
+      S2_.alloc((size_t)hx * lines * sizeof(Cx<T>));
+      rho_prev_.alloc(nvox_ * sizeof(T));
+      HIPC(hipMemset(rho_prev_.p, 0, rho_prev_.bytes));
+    }
+  }
+
+  // Per-workgroup source entry lists for each directional pass.
+  // psrc: density injection p/(ndim c^2) (first_order_steps.jl:561-580,
+  // 619-640, 692-715); usrc: raw velocity values (541-559, 586-617, 646-690).
+  void build_sources(const kw_sim_desc& d, bool het_c) {
+    if (d.n_psrc > 0 && d.psrc_index && d.psrc_signal) {
+      const long long ns = d.n_psrc, nc = std::max<long long>(d.psrc_ncols, 1);
+      std::vector<T> vals(ns * nc);
+      for (long long j = 0; j < ns; ++j) {
+        const long long idx = d.psrc_index[j];
+        if (idx < 0 || idx >= nvox_) raise(KW_ERR_INVALID, "pressure source index out of range");
+        const double c = het_c ? d.c0[idx] : d.c0_scalar;
+        const double sc = (double)nd_ * c * c;
+        for (long long k = 0; k < nc; ++k) vals[k * ns + j] = (T)(d.psrc_signal[j + ns * k] / sc);
+      }
+      upload(pval_, vals);
+      p_ns_ = ns;
+      p_nc_ = nc;
+      p_dir_ = d.p_mode == KW_DIRICHLET;
+      for (int a = 0; a < nd_; ++a) build_csr(a, d.psrc_index, ns, psrc_[a]);
+    }
+    if (d.n_usrc > 0 && d.usrc_index) {
+      for (int a = 0; a < nd_; ++a) {
+        if (!d.usrc_signal[a]) continue;
+        const long long ns = d.n_usrc, nc = std::max<long long>(d.usrc_ncols[a], 1);
+        std::vector<T> vals(ns * nc);
+        for (long long j = 0; j < ns; ++j) {
+          if (d.usrc_index[j] < 0 || d.usrc_index[j] >= nvox_)
+            raise(KW_ERR_INVALID, "velocity source index out of range");
+          for (long long k = 0; k < nc; ++k) vals[k * ns + j] = (T)d.usrc_signal[a][j + ns * k];
+        }
+        upload(uval_[a], vals);
+        u_ns_ = ns;
+        u_nc_[a] = nc;
+        build_csr(a, d.usrc_index, ns, usrc_[a]);
+      }
+      u_dir_ = d.u_mode == KW_DIRICHLET;
+    }
+  }
+
+  void build_csr(int axis, const int64_t* index, long long ns, SrcDev& out) {
+    const PassCfg& c = cfg_[axis];
+    const long long nwg = c.grid.x;
+    std::vector<long long> wg(ns);
+    std::vector<int> pos(ns);
+    const long long Nx = N_[0], Ny = N_[1];
+    const long long xblocks = ((Nx + 1) / 2 + c.lpw - 1) / c.lpw;
+    for (long long j = 0; j < ns; ++j) {
+      const long long idx = index[j];
+      const long long x = idx % Nx, y = (idx / Nx) % Ny, z = idx / (Nx * Ny);
+      long long w, cc, part, n;
+      if (axis == 0) {
+        const long long L = idx / Nx;
+        w = (L / 2) / c.lpw;
+        cc = (L / 2) % c.lpw;
+        part = L % 2;
+        n = x;
+      } else {
+        const long long xb = (x / 2) / c.lpw;
+        cc = (x / 2) % c.lpw;
+        part = x % 2;
+        const long long o = axis == 1 ? z : y;
+        n = axis == 1 ? y : z;
+        w = xb + xblocks * o;
+      }
+      wg[j] = w;
+      pos[j] = (int)(n | (part << 20) | (cc << 21));
+    }
+    std::vector<long long> order(ns);
+    for (long long j = 0; j < ns; ++j) order[j] = j;
+    std::stable_sort(order.begin(), order.end(), [&](long long a, long long b) { return wg[a] < wg[b]; });
+    std::vector<int> off(nwg + 1, 0), p(ns), r(ns);
+    for (long long j = 0; j < ns; ++j) off[wg[j] + 1]++;
+    for (long long w = 0; w < nwg; ++w) off[w + 1] += off[w];
+    for (long long k = 0; k < ns; ++k) {
+      p[k] = pos[order[k]];
+      r[k] = (int)order[k];
+    }
+    upload(out.off, off);
+    upload(out.pos, p);
+    upload(out.row, r);
+    out.any = true;
+  }
+
+  void build_sensor(const kw_sim_desc& d) {
+    nsens_ = d.n_sensor;
+    rec_ = d.record;
+    cartesian_ = d.n_sensor > 0 && d.sensor_index == nullptr;
+    if (tr_) {
+      // TR injection values p_TR[j, t] / (ndim c^2) (first_order.jl:234-242, 450-460, 678-689)
+      if (d.n_sensor > 0 && d.sensor_index) {
+        const long long n = d.n_sensor;
+        std::vector<T> vals(n * Nt_);
+        for (long long j = 0; j < n; ++j) {
+          const long long idx = d.sensor_index[j];
+          const double c = d.c0 ? d.c0[idx] : d.c0_scalar;
+          const double sc = (double)nd_ * c * c;
+          for (long long k = 0; k < Nt_; ++k) vals[k * n + j] = (T)(d.tr_data[j + n * k] / sc);
+        }
+        upload(tr_vals_, vals);
+      }
+    }
+    if (nsens_ == 0 || cartesian_) return;
+    std::vector<long long> idx(d.sensor_index, d.sensor_index + nsens_);
+    for (long long v : idx)
+      if (v < 0 || v >= nvox_) raise(KW_ERR_INVALID, "sensor index out of range");
+    upload(sidx_, idx);
+    auto fill_t = [&](DevBuf& b, long long n, double v) {
+      upload(b, std::vector<T>(n, (T)v));
+    };
+    auto fill_d = [&](DevBuf& b, long long n, double v) { upload(b, std::vector<double>(n, v)); };
+    if (rec_ & KW_REC_P) {
+      p_ts_.alloc((size_t)nsens_ * Nt_alloc_ * sizeof(T));
+      HIPC(hipMemset(p_ts_.p, 0, p_ts_.bytes));
+    }
+    for (int a = 0; a < 3; ++a) {
+      const uint32_t bit = a == 0 ? KW_REC_UX : (a == 1 ? KW_REC_UY : KW_REC_UZ);
+      if ((rec_ & bit) && a < nd_) {
+        u_ts_[a].alloc((size_t)nsens_ * Nt_alloc_ * sizeof(T));
+        HIPC(hipMemset(u_ts_[a].p, 0, u_ts_[a].bytes));
+      }
+    }
+    if (rec_ & KW_REC_P_MAX) fill_t(p_max_, nsens_, -INFINITY);
+    if (rec_ & KW_REC_P_MIN) fill_t(p_min_, nsens_, INFINITY);
+    if (rec_ & KW_REC_P_RMS) fill_d(p_rms_, nsens_, 0.0);
+    if (rec_ & KW_REC_P_FINAL) fill_t(p_final_, nsens_, 0.0);
+    if (rec_ & KW_REC_U_MAX) fill_d(u_max_, nsens_, -INFINITY);
+    if (rec_ & KW_REC_U_RMS) fill_d(u_rms_, nsens_, 0.0);
+    if (rec_ & KW_REC_U_FINAL) fill_d(u_final_, nsens_, 0.0);
+    if (rec_ & KW_REC_I_AVG) fill_d(I_avg_, nsens_, 0.0);
+    if (rec_ & KW_REC_I_MAX) fill_d(I_max_, nsens_, -INFINITY);
+    if ((rec_ & (KW_REC_I_AVG | KW_REC_I_MAX)) && sizeof(T) != 8)
+      raise(KW_ERR_INVALID,
+            "I_avg/I_max require data_cast=Float64 (_acoustic_intensity is typed for Float64)");
+  }
+
+  void build_args(const kw_sim_desc& d) {
+    const bool het_r = d.rho0 != nullptr;
+    for (int a = 0; a < nd_; ++a) {
+      PassArgs<T>& A = pa_[a];
+      std::memset(&A, 0, sizeof(A));
+      A.p = p_.as<T>();
+      A.u = u_[a].as<T>();
+      A.rho = rho_[a].as<T>();
+      A.rho0 = het_r ? rho0_.as<T>() : nullptr;
+      A.rho0_s = (T)d.rho0_scalar;
+      A.pml = pml_[a].as<T>();
+      A.pml_sg = pmlsg_[a].as<T>();
+      A.Dp = Dp_[a].as<Cx<T>>();
+      A.Dm = Dm_[a].as<Cx<T>>();
+      A.tw = tw_[a].as<Cx<T>>();
+      A.dt = (T)d.dt;
+      A.Nx = (int)N_[0];
+      A.n_lines = (int)(N_[1] * N_[2]);
+      const PassCfg& c = cfg_[a];
+      A.x_blocks = (int)(((N_[0] + 1) / 2 + c.lpw - 1) / c.lpw);
+      if (a == 1) {
+        A.Sa = N_[0];
+        A.So = N_[0] * N_[1];
+        A.n_outer = (int)N_[2];
+      } else if (a == 2) {
+        A.Sa = N_[0] * N_[1];
+        A.So = N_[0];
+        A.n_outer = (int)N_[1];
+      } else {
+        A.Sa = 1;
+      }
+      if (psrc_[a].any) {
+        A.psrc = SrcList{psrc_[a].off.as<int>(), psrc_[a].pos.as<int>(), psrc_[a].row.as<int>(), pval_.p,
+                         (int)p_ns_, (int)p_nc_, p_dir_ ? 1 : 0};
+      }
+      if (usrc_[a].any) {
+        A.usrc = SrcList{usrc_[a].off.as<int>(), usrc_[a].pos.as<int>(), usrc_[a].row.as<int>(), uval_[a].p,
+                         (int)u_ns_, (int)u_nc_[a], u_dir_ ? 1 : 0};
+      }
+    }
+    // the x pass carries the EOS input and its r2c stage
+    PassArgs<T>& X = pa_[0];
+    X.eos = 1;
+    X.rho_o1 = nd_ > 1 ? rho_[1].as<T>() : nullptr;
+    X.rho_o2 = nd_ > 2 ? rho_[2].as<T>() : nullptr;
+    X.c0sq = c0sq_.p ? c0sq_.as<T>() : nullptr;
+    X.nonlinear = d.nonlinear ? 1 : 0;
+    X.bona = bona_.p ? bona_.as<T>() : nullptr;
+    X.bona_s = (T)d.bona_scalar;
+    X.S1 = S1_.as<Cx<T>>();
+    X.S2 = stokes_ ? S2_.as<Cx<T>>() : nullptr;
+    X.rho_prev = stokes_ ? rho_prev_.as<T>() : nullptr;
+    X.inv_dt = (T)(1.0 / d.dt);
+
+    // spectral passes
+    const int hy = (int)(N_[1] / 2 + 1), hz = (int)(N_[2] / 2 + 1);
+    (void)hz;
+    if (nd_ > 1) {
+      SpecArgs<T>& Y = sa_[0];
+      std::memset(&Y, 0, sizeof(Y));
+      Y.S1 = S1_.as<Cx<T>>();
+      Y.S2 = stokes_ ? S2_.as<Cx<T>>() : nullptr;
+      Y.tw = tw_[1].as<Cx<T>>();
+      Y.M1 = M1_.as<T>();
+      Y.M2 = stokes_ ? M2_.as<T>() : nullptr;
+      Y.NH = (int)NH_;
+      Y.Sa = NH_;
+      Y.So = NH_ * N_[1];
+      Y.x_blocks = (int)((NH_ + spec_[0].lpw - 1) / spec_[0].lpw);
+      Y.axis = 1;
+      Y.Ny = (int)N_[1];
+      Y.Nz = (int)N_[2];
+      Y.MH1 = (int)NH_;
+      Y.MH2 = hy;
+    }
+    if (nd_ > 2) {
+      SpecArgs<T>& Z = sa_[1];
+      std::memset(&Z, 0, sizeof(Z));
+      Z.S1 = S1_.as<Cx<T>>();
+      Z.S2 = stokes_ ? S2_.as<Cx<T>>() : nullptr;
+      Z.tw = tw_[2].as<Cx<T>>();
+      Z.M1 = M1_.as<T>();
+      Z.M2 = stokes_ ? M2_.as<T>() : nullptr;
+      Z.NH = (int)NH_;
+      Z.Sa = NH_ * N_[1];
+      Z.So = NH_;
+      Z.x_blocks = (int)((NH_ + spec_[1].lpw - 1) / spec_[1].lpw);
+      Z.axis = 2;
+      Z.Ny = (int)N_[1];
+      Z.Nz = (int)N_[2];
+      Z.MH1 = (int)NH_;
+      Z.MH2 = hy;
+    }
+    std::memset(&ca_, 0, sizeof(ca_));
+    ca_.S1 = S1_.as<Cx<T>>();
+    ca_.S2 = (nd_ == 1 && stokes_) ? S2_.as<Cx<T>>() : nullptr;
+    ca_.M1 = nd_ == 1 ? M1_.as<T>() : nullptr;
+    ca_.M2 = (nd_ == 1 && stokes_) ? M2_.as<T>() : nullptr;
+    ca_.tw = tw_[0].as<Cx<T>>();
+    ca_.p = p_.as<T>();
+    ca_.n_lines = (int)(N_[1] * N_[2]);
+
+    // recording
+    std::memset(&ra_, 0, sizeof(ra_));
+    ra_.p = p_.as<T>();
+    for (int a = 0; a < 3; ++a) ra_.u[a] = a < nd_ ? u_[a].as<T>() : nullptr;
+    ra_.ndim = nd_;
+    ra_.idx = sidx_.as<long long>();
+    ra_.n = (int)nsens_;
+    ra_.Nt = (int)Nt_;
+    ra_.rec = rec_;
+    ra_.p_ts = p_ts_.as<T>();
+    for (int a = 0; a < 3; ++a) ra_.u_ts[a] = u_ts_[a].as<T>();
+    ra_.p_max = p_max_.as<T>();
+    ra_.p_min = p_min_.as<T>();
+    ra_.p_rms = p_rms_.as<double>();
+    ra_.p_final = p_final_.as<T>();
+    ra_.u_max = u_max_.as<double>();
+    ra_.u_rms = u_rms_.as<double>();
+    ra_.u_final = u_final_.as<double>();
+    ra_.I_avg = I_avg_.as<double>();
+    ra_.I_max = I_max_.as<double>();
+  }
+
+  // initialize_p0_{1,2,3}d! (first_order_steps.jl:58-191) with p0 already smoothed
+  void init_p0(const kw_sim_desc& d) {
+    if (d.smooth_p0) raise(KW_ERR_UNSUPPORTED, "smooth_p0 is applied by the host wrapper before kw_sim_create");
+    DevBuf p0;
+    upload(p0, to_t<T>(std::vector<double>(d.p0, d.p0 + nvox_)));
+    // rho_d = p0 / (ndim c0^2); p = p0
+    const unsigned nb = (unsigned)((nvox_ + 255) / 256);
+    HIPC(launch_init_rho<T>(nb, stream_, p0.as<T>(), c0sq_.p ? c0sq_.as<T>() : nullptr,
+                            (T)(d.c0_scalar * d.c0_scalar), (T)nd_, p_.as<T>(), rho_[0].as<T>(),
+                            nd_ > 1 ? rho_[1].as<T>() : nullptr, nd_ > 2 ? rho_[2].as<T>() : nullptr, nvox_));
+    // u_d = -dt/(2 rho0_sg) d/dd p0
+    for (int a = 0; a < nd_; ++a) {
+      PassArgs<T> A = pa_[a];
+      A.mode = 1;
+      A.eos = 0;
+      HIPC(launch_dir<T>(cfg_[a].n, a == 0, cfg_[a].grid, cfg_[a].block, cfg_[a].shmem, stream_, A));
+    }
+    HIPC(hipStreamSynchronize(stream_));
+  }
+
+  // ---- one time step ------------------------------------------------------
+  void step(Prof* pr) {
+    const int t = (int)(++t_);
+    if (tr_ && tr_vals_.p && nsens_ > 0) {
+      const long long n = nsens_;
+      const T* v = tr_vals_.as<T>() + (Nt_ - t) * n;
+      launch(pr, C_TR, [&] {
+        HIPC(launch_tr_inject<T>((unsigned)((n + 255) / 256), stream_, rho_[0].as<T>(),
+                                 nd_ > 1 ? rho_[1].as<T>() : nullptr, nd_ > 2 ? rho_[2].as<T>() : nullptr, sidx_tr(),
+                                 v, (int)n));
+      });
+    }
+    for (int a = nd_ - 1; a >= 1; --a) {
+      pa_[a].t = t;
+      launch(pr, a == 1 ? C_DIR_Y : C_DIR_Z,
+             [&] { HIPC(launch_dir<T>(cfg_[a].n, false, cfg_[a].grid, cfg_[a].block, cfg_[a].shmem, stream_, pa_[a])); });
+    }
+    pa_[0].t = t;
+    launch(pr, C_DIR_X,
+           [&] { HIPC(launch_dir<T>(cfg_[0].n, true, cfg_[0].grid, cfg_[0].block, cfg_[0].shmem, stream_, pa_[0])); });
+    if (nd_ == 3) {
+      SpecArgs<T> Y = sa_[0];
+      Y.mode = 0;
+      launch(pr, C_SPEC_Y_FWD, [&] { HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y)); });
+      SpecArgs<T> Z = sa_[1];
+      Z.mode = 2;
+      launch(pr, C_SPEC_Z, [&] { HIPC(launch_spec<T>(spec_[1].n, spec_[1].grid, spec_[1].block, spec_[1].shmem, stream_, Z)); });
+      Y.mode = 1;
+      Y.S2 = nullptr;
+      launch(pr, C_SPEC_Y_INV, [&] { HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y)); });
+    } else if (nd_ == 2) {
+      SpecArgs<T> Y = sa_[0];
+      Y.mode = 2;
+      launch(pr, C_SPEC_Y, [&] { HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y)); });
+    }
+    launch(pr, C_C2R, [&] { HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, ca_)); });
+    if (nsens_ > 0 && !cartesian_ && rec_) {
+      ra_.t = t;
+      launch(pr, C_REC, [&] {
+        HIPC(launch_record<T>((unsigned)((nsens_ + 255) / 256), stream_, ra_));
+      });
+    }
+  }
+
+  const long long* sidx_tr() {
+    if (!sidx_.p) {
+      // TR with nothing recorded still needs the injection indices
+      raise(KW_ERR_STATE, "time reversal needs a binary sensor mask");
+    }
+    return sidx_.as<long long>();
+  }
+
+  void finalize() {
+    finalized_ = true;
+    if (nsens_ == 0 || cartesian_) return;
+    const unsigned nb = (unsigned)((nsens_ + 255) / 256);
+    if (rec_ & KW_REC_P_RMS) HIPC(launch_finalize_rms(nb, stream_, p_rms_.as<double>(), nsens_, (double)Nt_));
+    if (rec_ & KW_REC_U_RMS) HIPC(launch_finalize_rms(nb, stream_, u_rms_.as<double>(), nsens_, (double)Nt_));
+  }
+
+  int dev_;
+  hipStream_t stream_ = nullptr;
+  int nd_ = 0;
+  long long N_[3] = {1, 1, 1};
+  long long nvox_ = 0, NH_ = 0;
+  double dt_ = 0;
+  long long Nt_ = 0, Nt_alloc_ = 0, t_ = 0;
+  bool tr_ = false, stokes_ = false, finalized_ = false, cartesian_ = false;
+  DevBuf p_, u_[3], rho_[3], rho0_, c0sq_, bona_, S1_, S2_, rho_prev_, M1_, M2_;
+  DevBuf tw_[3], Dp_[3], Dm_[3], pml_[3], pmlsg_[3];
+  PassCfg cfg_[3], spec_[2], c2r_;
+  PassArgs<T> pa_[3];
+  SpecArgs<T> sa_[2];
+  C2RArgs<T> ca_;
+  RecArgs<T> ra_;
+  // sources
+  SrcDev psrc_[3], usrc_[3];
+  DevBuf pval_, uval_[3];
+  long long p_ns_ = 0, p_nc_ = 1, u_ns_ = 0, u_nc_[3] = {1, 1, 1};
+  bool p_dir_ = false, u_dir_ = false;
+  // sensor
+  long long nsens_ = 0;
+  uint32_t rec_ = 0;
+  DevBuf sidx_, p_ts_, u_ts_[3], p_max_, p_min_, p_rms_, p_final_, u_max_, u_rms_, u_final_, I_avg_, I_max_;
+  DevBuf tr_vals_;
+};
+
+template <typename T>
+constexpr const char* Sim<T>::kClassNames[];
+
+}  // namespace
+
+struct kw_ctx {
+  int device;
+};
+struct kw_sim {
+  std::unique_ptr<SimBase> impl;
+};
+
+#define KW_GUARD_BEGIN try {
+#define KW_GUARD_END                      \
+  }                                       \
+  catch (const KwError& e) {              \
+    g_err = e.msg;                        \
+    return e.code;                        \
+  }                                       \
+  catch (const std::exception& e) {       \
+    g_err = e.what();                     \
+    return KW_ERR_INVALID;                \
+  }                                       \
+  catch (...) {                           \
+    g_err = "unknown error";              \
+    return KW_ERR_INVALID;                \
+  }
+
+extern "C" {
+
+int32_t kw_abi_version(void) { return KW_ABI_VERSION; }
+
+const char* kw_last_error(void) { return g_err.c_str(); }
+
+int32_t kw_device_count(int32_t* count) {
+  KW_GUARD_BEGIN
+  if (!count) raise(KW_ERR_INVALID, "null count");
+  int n = 0;
+  HIPC(hipGetDeviceCount(&n));
+  *count = n;
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_ctx_create(int32_t device, kw_ctx** out) {
+  KW_GUARD_BEGIN
+  if (!out) raise(KW_ERR_INVALID, "null output pointer");
+  int n = 0;
+  HIPC(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) raise(KW_ERR_INVALID, "device index out of range");
+  HIPC(hipSetDevice(device));
+  *out = new kw_ctx{device};
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_ctx_destroy(kw_ctx* ctx) {
+  delete ctx;
+  return KW_OK;
+}
+
+int32_t kw_fft_length_supported(int64_t n) { return size_supported(n) ? 1 : 0; }
+
+int32_t kw_sim_create(kw_ctx* ctx, const kw_sim_desc* desc, kw_sim** out) {
+  KW_GUARD_BEGIN
+  if (!ctx || !desc || !out) raise(KW_ERR_INVALID, "null argument");
+  std::unique_ptr<kw_sim> s(new kw_sim);
+  if (desc->precision == 4)
+    s->impl.reset(new Sim<float>(ctx->device, *desc));
+  else if (desc->precision == 8)
+    s->impl.reset(new Sim<double>(ctx->device, *desc));
+  else
+    raise(KW_ERR_INVALID, "precision must be 4 or 8");
+  *out = s.release();
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_sim_run(kw_sim* sim, int64_t n_steps) {
+  KW_GUARD_BEGIN
+  if (!sim) raise(KW_ERR_INVALID, "null sim");
+  if (n_steps < 0) raise(KW_ERR_INVALID, "negative step count");
+  sim->impl->run(n_steps);
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_sim_synchronize(kw_sim* sim) {
+  KW_GUARD_BEGIN
+  if (!sim) raise(KW_ERR_INVALID, "null sim");
+  sim->impl->sync();
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_sim_steps_done(const kw_sim* sim, int64_t* steps) {
+  KW_GUARD_BEGIN
+  if (!sim || !steps) raise(KW_ERR_INVALID, "null argument");
+  *steps = sim->impl->steps_done();
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_sim_get_field(kw_sim* sim, int32_t which, double* out) {
+  KW_GUARD_BEGIN
+  if (!sim || !out) raise(KW_ERR_INVALID, "null argument");
+  sim->impl->get_field(which, out);
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_sim_sensor_shape(kw_sim* sim, uint32_t field, int64_t* rows, int64_t* cols) {
+  KW_GUARD_BEGIN
+  if (!sim || !rows || !cols) raise(KW_ERR_INVALID, "null argument");
+  long long r = 0, c = 0;
+  sim->impl->sensor_shape(field, &r, &c);
+  *rows = r;
+  *cols = c;
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_sim_get_sensor(kw_sim* sim, uint32_t field, double* out) {
+  KW_GUARD_BEGIN
+  if (!sim || !out) raise(KW_ERR_INVALID, "null argument");
+  sim->impl->get_sensor(field, out);
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_sim_profile(kw_sim* sim, int64_t n_steps, int32_t max_classes, int32_t* n_classes, const char** names,
+                       double* ms, int64_t* launches) {
+  KW_GUARD_BEGIN
+  if (!sim || !n_classes || !names || !ms || !launches) raise(KW_ERR_INVALID, "null argument");
+  int nc = 0;
+  std::vector<long long> l(max_classes > 0 ? max_classes : 1);
+  sim->impl->profile(n_steps, max_classes, &nc, names, ms, l.data());
+  for (int i = 0; i < nc; ++i) launches[i] = l[i];
+  *n_classes = nc;
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_sim_destroy(kw_sim* sim) {
+  KW_GUARD_BEGIN
+  delete sim;
+  return KW_OK;
+  KW_GUARD_END
+}
+
+}  // extern "C"

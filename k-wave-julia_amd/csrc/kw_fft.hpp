@@ -1,0 +1,335 @@
+// In-register / LDS line FFT engine for gfx950 (CDNA4, wave64).
+//
+// A "line" of N complex values is owned by TPL consecutive threads of a
+// workgroup.  Thread t holds v[q] = x[t + TPL*q] for q < E = N/TPL (the
+// "stride-TPL" layout).  A mixed-radix Stockham autosort transform with the
+// compile-time radix list Rs... runs the butterflies of every pass in
+// registers; passes exchange through the line's LDS slice.  The last pass
+// writes its outputs straight back into registers, and because the Stockham
+// autosort output of the final pass lands at j + r*N/R, the output layout is
+// the same stride-TPL layout as the input.  Forward and inverse transforms
+// therefore chain without any reordering, which is what lets the pass
+// kernels do  FFT -> pointwise multiply -> IFFT  with one LDS exchange per
+// transform for N = R0*R1.
+//
+// Forward uses W = exp(-2 pi i / N); the inverse is unnormalised (callers
+// fold 1/N into their spectral multipliers).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace kw {
+
+template <typename T>
+struct Cx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ Cx<T> cadd(Cx<T> a, Cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ Cx<T> csub(Cx<T> a, Cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ Cx<T> cmul(Cx<T> a, Cx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ Cx<T> cmulc(Cx<T> a, Cx<T> b) {
+  return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename T>
+__device__ __forceinline__ Cx<T> cscale(Cx<T> a, T s) { return {a.x * s, a.y * s}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename T, bool INV>
+__device__ __forceinline__ Cx<T> rot90(Cx<T> a) {
+  return INV ? Cx<T>{-a.y, a.x} : Cx<T>{a.y, -a.x};
+}
+
+// ---------------------------------------------------------------------------
+// Compile-time trigonometry for the in-register butterfly constants.
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+constexpr double ce_sin_red(double x) {  // |x| <= pi/4, Taylor to 1e-20
+  double term = x, sum = x, x2 = x * x;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x2 / ((2 * n) * (2 * n + 1));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ce_cos_red(double x) {
+  double term = 1.0, sum = 1.0, x2 = x * x;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x2 / ((2 * n - 1) * (2 * n));
+    sum += term;
+  }
+  return sum;
+}
+// cos/sin of 2*pi*num/den with exact octant reduction on the rational angle
+constexpr void ce_sincos_frac(long num, long den, double& s, double& c) {
+  num %= den;
+  if (num < 0) num += den;
+  // angle = 2 pi num/den; octant = floor(8 num / den)
+  long o = (8 * num) / den;
+  // remainder angle in [0, pi/4): r = 2pi*num/den - o*pi/4
+  double r = 2.0 * kPi * (double)(8 * num - o * den) / (8.0 * (double)den);
+  double sr = ce_sin_red(r), cr = ce_cos_red(r);
+  // rotate by o*pi/4 using exact octant identities
+  const double h = 0.70710678118654752440084436210484904;
+  double c8[8] = {1, h, 0, -h, -1, -h, 0, h};
+  double s8[8] = {0, h, 1, h, 0, -h, -1, -h};
+  c = cr * c8[o] - sr * s8[o];
+  s = sr * c8[o] + cr * s8[o];
+}
+
+template <int R>
+struct RootTable {
+  double c[R > 0 ? R : 1];
+  double s[R > 0 ? R : 1];
+  constexpr RootTable() : c(), s() {
+    for (int k = 0; k < R; ++k) {
+      double sv = 0, cv = 0;
+      ce_sincos_frac(k, R, sv, cv);
+      c[k] = cv;
+      s[k] = sv;
+    }
+  }
+};
+
+// W_R^m = exp(-2 pi i m / R) (forward), conjugate for the inverse.
+template <typename T, int R, bool INV>
+__device__ __forceinline__ Cx<T> root(int m) {
+  constexpr RootTable<R> tab{};
+  m %= R;
+  return Cx<T>{(T)tab.c[m], INV ? (T)tab.s[m] : (T)(-tab.s[m])};
+}
+
+// ---------------------------------------------------------------------------
+// Small DFTs in registers (natural order in, natural order out).
+// ---------------------------------------------------------------------------
+template <int R>
+struct SmallestFactor {
+  static constexpr int f() {
+    for (int p = 2; p * p <= R; ++p)
+      if (R % p == 0) return (p == 2 && R % 4 == 0 && R > 4) ? 4 : p;
+    return R;
+  }
+  static constexpr int value = f();
+};
+
+template <typename T, int R, bool INV>
+struct DFT {
+  static constexpr int A = SmallestFactor<R>::value;  // outer radix
+  static constexpr int B = R / A;
+  __device__ __forceinline__ static void run(Cx<T> (&x)[R]) {
+    if constexpr (A == R) {
+      // prime radix > 5: direct O(R^2) evaluation with folded constants
+      Cx<T> y[R];
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        Cx<T> acc = x[0];
+#pragma unroll
+        for (int n = 1; n < R; ++n) acc = cadd(acc, cmul(x[n], root<T, R, INV>(n * k)));
+        y[k] = acc;
+      }
+#pragma unroll
+      for (int k = 0; k < R; ++k) x[k] = y[k];
+    } else {
+      // Cooley-Tukey: n = n1 + A*n2, k = k2 + B*k1
+      Cx<T> y[A][B];
+#pragma unroll
+      for (int n1 = 0; n1 < A; ++n1) {
+        Cx<T> col[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) col[n2] = x[n1 + A * n2];
+        DFT<T, B, INV>::run(col);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2)
+          y[n1][k2] = (n1 * k2 == 0) ? col[k2] : cmul(col[k2], root<T, R, INV>(n1 * k2));
+      }
+#pragma unroll
+      for (int k2 = 0; k2 < B; ++k2) {
+        Cx<T> row[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) row[n1] = y[n1][k2];
+        DFT<T, A, INV>::run(row);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) x[k2 + B * k1] = row[k1];
+      }
+    }
+  }
+};
+
+template <typename T, bool INV>
+struct DFT<T, 1, INV> {
+  __device__ __forceinline__ static void run(Cx<T> (&)[1]) {}
+};
+template <typename T, bool INV>
+struct DFT<T, 2, INV> {
+  __device__ __forceinline__ static void run(Cx<T> (&x)[2]) {
+    Cx<T> a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+  }
+};
+template <typename T, bool INV>
+struct DFT<T, 3, INV> {
+  __device__ __forceinline__ static void run(Cx<T> (&x)[3]) {
+    const T c1 = (T)-0.5;
+    const T s1 = (T)(INV ? 0.86602540378443864676372317075294 : -0.86602540378443864676372317075294);
+    Cx<T> t1 = cadd(x[1], x[2]);
+    Cx<T> t2 = csub(x[1], x[2]);
+    Cx<T> m = {x[0].x + c1 * t1.x, x[0].y + c1 * t1.y};
+    // s1 * i * t2
+    Cx<T> r = {-s1 * t2.y, s1 * t2.x};
+    x[0] = cadd(x[0], t1);
+    x[1] = cadd(m, r);
+    x[2] = csub(m, r);
+  }
+};
+template <typename T, bool INV>
+struct DFT<T, 4, INV> {
+  __device__ __forceinline__ static void run(Cx<T> (&x)[4]) {
+    Cx<T> a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+    Cx<T> b0 = cadd(x[1], x[3]), b1 = rot90<T, INV>(csub(x[1], x[3]));
+    x[0] = cadd(a0, b0);
+    x[2] = csub(a0, b0);
+    x[1] = cadd(a1, b1);
+    x[3] = csub(a1, b1);
+  }
+};
+template <typename T, bool INV>
+struct DFT<T, 5, INV> {
+  __device__ __forceinline__ static void run(Cx<T> (&x)[5]) {
+    const T c1 = (T)0.30901699437494742410229341718282;   // cos(2pi/5)
+    const T c2 = (T)-0.80901699437494742410229341718282;  // cos(4pi/5)
+    const T sg = INV ? (T)1 : (T)-1;
+    const T s1 = sg * (T)0.95105651629515357211643933337938;  // sin(2pi/5)
+    const T s2 = sg * (T)0.58778525229247312916870595463907;  // sin(4pi/5)
+    Cx<T> a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
+    Cx<T> a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
+    Cx<T> m1 = {x[0].x + c1 * a1.x + c2 * a2.x, x[0].y + c1 * a1.y + c2 * a2.y};
+    Cx<T> m2 = {x[0].x + c2 * a1.x + c1 * a2.x, x[0].y + c2 * a1.y + c1 * a2.y};
+    // i*(s1*b1 + s2*b2), i*(s2*b1 - s1*b2)
+    Cx<T> n1 = {-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x};
+    Cx<T> n2 = {-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x};
+    x[0] = cadd(x[0], cadd(a1, a2));
+    x[1] = cadd(m1, n1);
+    x[4] = csub(m1, n1);
+    x[2] = cadd(m2, n2);
+    x[3] = csub(m2, n2);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Line plans: threads per line and the radix list for each supported N.
+// ---------------------------------------------------------------------------
+template <int... Rs>
+struct Radices {};
+
+template <int N>
+struct Plan;
+#define KW_PLAN(N_, TPL_, ...)                     \
+  template <>                                      \
+  struct Plan<N_> {                                \
+    static constexpr int N = N_;                   \
+    static constexpr int TPL = TPL_;               \
+    static constexpr int E = N_ / TPL_;            \
+    using R = Radices<__VA_ARGS__>;                \
+  };
+KW_PLAN(8, 1, 8)
+KW_PLAN(12, 1, 12)
+KW_PLAN(16, 1, 16)
+KW_PLAN(24, 2, 12, 2)
+KW_PLAN(32, 4, 8, 4)
+KW_PLAN(48, 4, 12, 4)
+KW_PLAN(64, 8, 8, 8)
+KW_PLAN(96, 4, 24, 4)
+KW_PLAN(128, 8, 16, 8)
+KW_PLAN(192, 8, 24, 8)
+KW_PLAN(256, 16, 16, 16)
+KW_PLAN(384, 16, 24, 8, 2)
+KW_PLAN(512, 16, 32, 16)
+KW_PLAN(768, 32, 24, 8, 4)
+KW_PLAN(1024, 32, 32, 32)
+KW_PLAN(2048, 64, 32, 8, 8)
+KW_PLAN(4096, 128, 32, 16, 8)
+#undef KW_PLAN
+
+// LDS slice of one line: one complex of padding per 16 keeps the
+// stride-16/32 Stockham writes off a single bank group.  The 4096 plan is
+// left unpadded so a double-precision line still fits a 64 KiB workgroup.
+template <int N>
+struct LdsLayout {
+  static constexpr bool PAD = N >= 16 && N < 4096;
+  static constexpr int PITCH = PAD ? N + N / 16 + 1 : N;
+  __device__ __forceinline__ static int at(int i) { return PAD ? i + (i >> 4) : i; }
+};
+
+template <typename T, int N, int TPL, bool INV, int Ns, int R, int... Rest>
+struct StockhamPass {
+  static constexpr int E = N / TPL;
+  static constexpr int NB = E / R;
+  static_assert(E % R == 0, "radix must divide the per-thread element count");
+  __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Cx<T>* __restrict__ tw,
+                                             int t) {
+    constexpr bool LAST = sizeof...(Rest) == 0;
+    static_assert(!LAST || Ns * R == N, "radix product must equal N");
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+      Cx<T> x[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) x[r] = v[m + r * NB];
+      const int j = t + TPL * m;
+      if constexpr (Ns > 1) {
+        const int jm = j % Ns;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          Cx<T> w = tw[jm * r * (N / (Ns * R))];
+          x[r] = INV ? cmulc(x[r], w) : cmul(x[r], w);
+        }
+      }
+      DFT<T, R, INV>::run(x);
+      if constexpr (LAST) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[m + r * NB] = x[r];
+      } else {
+        const int base = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds[LdsLayout<N>::at(base + r * Ns)] = x[r];
+      }
+    }
+    if constexpr (!LAST) {
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = lds[LdsLayout<N>::at(t + TPL * q)];
+      __syncthreads();
+      StockhamPass<T, N, TPL, INV, Ns * R, Rest...>::run(v, lds, tw, t);
+    }
+  }
+};
+
+template <typename T, int N, bool INV, typename RL>
+struct LineFFTImpl;
+template <typename T, int N, bool INV, int... Rs>
+struct LineFFTImpl<T, N, INV, Radices<Rs...>> {
+  static constexpr int TPL = Plan<N>::TPL;
+  static constexpr int E = Plan<N>::E;
+  __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Cx<T>* __restrict__ tw,
+                                             int t) {
+    StockhamPass<T, N, TPL, INV, 1, Rs...>::run(v, lds, tw, t);
+  }
+};
+
+// Public entry: forward (INV=false) or unnormalised inverse transform of the
+// line held in v (stride-TPL layout), using the line's LDS slice `lds`.
+// Every thread of the workgroup must call it (it contains barriers).
+template <typename T, int N, bool INV>
+__device__ __forceinline__ void line_fft(Cx<T> (&v)[Plan<N>::E], Cx<T>* lds, const Cx<T>* __restrict__ tw,
+                                         int t) {
+  LineFFTImpl<T, N, INV, typename Plan<N>::R>::run(v, lds, tw, t);
+}
+
+}  // namespace kw

@@ -1,0 +1,677 @@
+// Time-step kernels of the k-space first-order PSTD loop for gfx950.
+//
+// One time step (first_order_steps.jl:296-535) is split into passes over
+// lines of the grid:
+//   * a directional pass per axis d (y, z, then x): for every line along d
+//     it computes  dp/dd -> u_d update -> [u source] -> du_d/dd -> rho_d
+//     update -> [p source]  entirely in registers/LDS.  This is exact because
+//     the staggered derivative multiplier i k_d exp(+-i k_d h/2) depends on
+//     k_d only and is Hermitian, so the N-D rfft/irfft of spectral_gradient!
+//     (fft_utils.jl:126-182) equals a 1-D transform along d.
+//   * the x pass also forms the equation-of-state input (c0^2) rho(1 + B/A
+//     rho/(2 rho0)) and does the real-to-complex stage of the EOS transform;
+//   * spectral passes along y/z (3-D: y fwd, z fwd*M*inv, y inv; 2-D: y
+//     fwd*M*inv) with M = (c^2) kappa (1 + nabla1) / N read from an octant
+//     table; and a complex-to-real pass along x that writes p.
+// Two real lines travel as one complex line (re = line a, im = line b);
+// the derivative and EOS multipliers are Hermitian so the pair separates
+// exactly on the way back.
+#pragma once
+#include "kw_fft.hpp"
+
+namespace kw {
+
+// ---------------------------------------------------------------------------
+// Arguments
+// ---------------------------------------------------------------------------
+struct SrcList {
+  const int* off;   // per-workgroup CSR offsets (n_wg + 1) or null
+  const int* pos;   // packed (line c, part, element n): n | part<<20 | c<<21
+  const int* row;   // source row j
+  const void* val;  // T values, time-major [col][n_src]
+  int n_src;
+  int ncols;
+  int dirichlet;
+};
+
+template <typename T>
+struct PassArgs {
+  // fields of this direction
+  const T* p;
+  T* u;
+  T* rho;
+  // medium
+  const T* rho0;  // map or null
+  T rho0_s;
+  // line tables (length N of this axis)
+  const T* pml;
+  const T* pml_sg;
+  const Cx<T>* Dp;  // i k exp(+i k h/2) / N
+  const Cx<T>* Dm;  // i k exp(-i k h/2) / N
+  const Cx<T>* tw;  // exp(-2 pi i m / N)
+  T dt;
+  // geometry
+  int Nx;          // line count along x pairing (strided) / line length (contig)
+  long long Sa;    // element stride along the line
+  long long So;    // stride of the outer index
+  int n_outer;     // outer count
+  int n_lines;     // contiguous: number of real lines
+  int x_blocks;    // strided: workgroups along x
+  // sources
+  SrcList psrc;
+  SrcList usrc;
+  int t;     // 1-based time index
+  int mode;  // 0 = step, 1 = p0 init (u only, dt/2, no PML)
+  // fused EOS (x pass only)
+  int eos;             // 0 none, 1 fused r2c
+  const T* rho_o1;     // other split density components (null if absent)
+  const T* rho_o2;
+  const T* c0sq;       // c0^2 map or null (homogeneous: folded into M)
+  const T* bona;       // B/A map or null
+  T bona_s;            // scalar B/A (when bona == null)
+  int nonlinear;
+  Cx<T>* S1;           // half spectrum (Nx/2+1, Ny, Nz)
+  Cx<T>* S2;           // second spectrum for the homogeneous :stokes branch
+  T* rho_prev;         // previous total density (:stokes)
+  T inv_dt;
+};
+
+// ---------------------------------------------------------------------------
+// Line indexing
+// ---------------------------------------------------------------------------
+// Contiguous (x) lines: thread -> (t = tid % TPL, c = tid / TPL); pair pi =
+// blockIdx.x * LPW + c holds real lines 2pi (re) and 2pi+1 (im).
+// Strided lines: thread -> (c = tid % LPW, t = tid / LPW); the workgroup
+// covers x in [2*LPW*xb, 2*LPW*(xb+1)) at outer index o; pair c holds x =
+// x0 + 2c (re) and x0 + 2c + 1 (im).
+
+template <typename T, int N>
+struct LineCtx {
+  static constexpr int TPL = Plan<N>::TPL;
+  static constexpr int E = Plan<N>::E;
+  int t, c;
+  bool va, vb;          // real line a / b valid
+  long long ba, bb;     // base offsets of lines a / b
+  long long sa;         // element stride
+  __device__ __forceinline__ long long ia(int q) const { return ba + sa * (long long)(t + TPL * q); }
+  __device__ __forceinline__ long long ib(int q) const { return bb + sa * (long long)(t + TPL * q); }
+  __device__ __forceinline__ int n(int q) const { return t + TPL * q; }
+
+  __device__ __forceinline__ void load(const T* __restrict__ f, Cx<T> (&v)[E]) const {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      v[q].x = va ? f[ia(q)] : (T)0;
+      v[q].y = vb ? f[ib(q)] : (T)0;
+    }
+  }
+  __device__ __forceinline__ void store(T* __restrict__ f, const Cx<T> (&v)[E]) const {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      if (va) f[ia(q)] = v[q].x;
+      if (vb) f[ib(q)] = v[q].y;
+    }
+  }
+  // value at element n+1 (periodic) along the line
+  __device__ __forceinline__ void load_next(const T* __restrict__ f, Cx<T> (&v)[E]) const {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      int nn = t + TPL * q + 1;
+      nn = (nn == N) ? 0 : nn;
+      v[q].x = va ? f[ba + sa * (long long)nn] : (T)0;
+      v[q].y = vb ? f[bb + sa * (long long)nn] : (T)0;
+    }
+  }
+};
+
+template <typename T, int N, bool CONTIG>
+__device__ __forceinline__ LineCtx<T, N> make_ctx(const PassArgs<T>& a, int& lpw_out) {
+  constexpr int TPL = Plan<N>::TPL;
+  LineCtx<T, N> L;
+  const int tid = threadIdx.x;
+  const int lpw = blockDim.x / TPL;
+  lpw_out = lpw;
+  if constexpr (CONTIG) {
+    L.t = tid % TPL;
+    L.c = tid / TPL;
+    const long long pi = (long long)blockIdx.x * lpw + L.c;
+    const long long la = 2 * pi, lb = 2 * pi + 1;
+    L.va = la < a.n_lines;
+    L.vb = lb < a.n_lines;
+    L.ba = la * (long long)N;
+    L.bb = lb * (long long)N;
+    L.sa = 1;
+  } else {
+    L.c = tid % lpw;
+    L.t = tid / lpw;
+    const int xb = blockIdx.x % a.x_blocks;
+    const int o = blockIdx.x / a.x_blocks;
+    const int x = 2 * (xb * lpw + L.c);
+    L.va = x < a.Nx;
+    L.vb = x + 1 < a.Nx;
+    L.ba = (long long)x + a.So * o;
+    L.bb = L.ba + 1;
+    L.sa = a.Sa;
+  }
+  return L;
+}
+
+// Apply the workgroup's source entries to the pair-line registers through LDS.
+// pos packs (c << 21) | (part << 20) | n.
+template <typename T, int N>
+__device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)[Plan<N>::E], Cx<T>* lds_all,
+                                       const LineCtx<T, N>& L) {
+  if (s.off == nullptr) return;
+  const int e0 = s.off[blockIdx.x], e1 = s.off[blockIdx.x + 1];
+  if (e0 >= e1) return;  // uniform across the workgroup
+  constexpr int E = Plan<N>::E;
+  constexpr int P = LdsLayout<N>::PITCH;
+  Cx<T>* lds = lds_all + L.c * P;
+#pragma unroll
+  for (int q = 0; q < E; ++q) lds[LdsLayout<N>::at(L.n(q))] = v[q];
+  __syncthreads();
+  const int col = (t_index < s.ncols ? t_index : s.ncols) - 1;
+  const T* vals = reinterpret_cast<const T*>(s.val) + (long long)col * s.n_src;
+  for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+    const int pk = s.pos[e];
+    const int n = pk & 0xFFFFF, part = (pk >> 20) & 1, c = pk >> 21;
+    const T val = vals[s.row[e]];
+    Cx<T>& cell = lds_all[c * P + LdsLayout<N>::at(n)];
+    T& dst = part ? cell.y : cell.x;
+    dst = s.dirichlet ? val : dst + val;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = lds[LdsLayout<N>::at(L.n(q))];
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Directional pass (contiguous x lines or strided y/z lines)
+// ---------------------------------------------------------------------------
+template <typename T, int N, bool CONTIG>
+__global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
+  constexpr int E = Plan<N>::E;
+  constexpr int P = LdsLayout<N>::PITCH;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
+  int lpw;
+  const LineCtx<T, N> L = make_ctx<T, N, CONTIG>(a, lpw);
+  Cx<T>* lds = lds_all + L.c * P;
+
+  // --- grad p (staggered +) : rfft -> i k e^{+ikh/2} -> irfft
+  Cx<T> v[E];
+  L.load(a.p, v);
+  line_fft<T, N, false>(v, lds, a.tw, L.t);
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dp[L.n(q)]);
+  line_fft<T, N, true>(v, lds, a.tw, L.t);
+
+  // --- velocity update  u = pml_sg (pml_sg u - dt/rho0_sg dp)
+  Cx<T> r0[E];
+  if (a.rho0 != nullptr) {
+    Cx<T> r1[E];
+    L.load(a.rho0, r0);
+    L.load_next(a.rho0, r1);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      r1[q].x = (r0[q].x + r1[q].x) * (T)0.5;
+      r1[q].y = (r0[q].y + r1[q].y) * (T)0.5;
+    }
+    if (a.mode == 1) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const T h = a.dt * (T)0.5;
+        v[q].x = -h / r1[q].x * v[q].x;
+        v[q].y = -h / r1[q].y * v[q].y;
+      }
+    } else {
+      Cx<T> w[E];
+      L.load(a.u, w);
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const T s = a.pml_sg[L.n(q)];
+        v[q].x = s * (s * w[q].x - a.dt / r1[q].x * v[q].x);
+        v[q].y = s * (s * w[q].y - a.dt / r1[q].y * v[q].y);
+      }
+    }
+  } else {
+    if (a.mode == 1) {
+      const T f = -(a.dt * (T)0.5) / a.rho0_s;
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = cscale(v[q], f);
+    } else {
+      Cx<T> w[E];
+      L.load(a.u, w);
+      const T f = a.dt / a.rho0_s;
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const T s = a.pml_sg[L.n(q)];
+        v[q].x = s * (s * w[q].x - f * v[q].x);
+        v[q].y = s * (s * w[q].y - f * v[q].y);
+      }
+    }
+  }
+  if (a.mode == 1) {
+    L.store(a.u, v);
+    return;
+  }
+  inject<T, N>(a.usrc, a.t, v, lds_all, L);
+  L.store(a.u, v);
+
+  // --- div u (staggered -)
+  line_fft<T, N, false>(v, lds, a.tw, L.t);
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dm[L.n(q)]);
+  line_fft<T, N, true>(v, lds, a.tw, L.t);
+
+  // --- density update  rho = pml (pml rho - dt rho0 du)
+  {
+    Cx<T> w[E];
+    L.load(a.rho, w);
+    if (a.rho0 != nullptr) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const T s = a.pml[L.n(q)];
+        w[q].x = s * (s * w[q].x - a.dt * r0[q].x * v[q].x);
+        w[q].y = s * (s * w[q].y - a.dt * r0[q].y * v[q].y);
+      }
+    } else {
+      const T f = a.dt * a.rho0_s;
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const T s = a.pml[L.n(q)];
+        w[q].x = s * (s * w[q].x - f * v[q].x);
+        w[q].y = s * (s * w[q].y - f * v[q].y);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = w[q];
+  }
+  inject<T, N>(a.psrc, a.t, v, lds_all, L);
+  L.store(a.rho, v);
+
+  if constexpr (CONTIG) {
+    if (a.eos == 0) return;
+    // --- EOS input f = (c0^2) rho (1 + B/A/2 rho/rho0), then r2c along x
+    Cx<T> f[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) f[q] = v[q];
+    if (a.rho_o1) {
+      Cx<T> w[E];
+      L.load(a.rho_o1, w);
+#pragma unroll
+      for (int q = 0; q < E; ++q) f[q] = cadd(f[q], w[q]);
+    }
+    if (a.rho_o2) {
+      Cx<T> w[E];
+      L.load(a.rho_o2, w);
+#pragma unroll
+      for (int q = 0; q < E; ++q) f[q] = cadd(f[q], w[q]);
+    }
+    Cx<T> g[E];  // second spectrum input (:stokes)
+    if (a.S2 != nullptr) {
+      Cx<T> pr[E];
+      L.load(a.rho_prev, pr);
+      L.store(a.rho_prev, f);
+      if (a.nonlinear) {
+        Cx<T> bb[E];
+        if (a.bona) L.load(a.bona, bb);
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+          const T ba = a.bona ? bb[q].x : a.bona_s, bbv = a.bona ? bb[q].y : a.bona_s;
+          const T ra = a.rho0 ? r0[q].x : a.rho0_s, rb = a.rho0 ? r0[q].y : a.rho0_s;
+          const T ea = f[q].x * ((T)1 + ba / (T)2 * f[q].x / ra);
+          const T eb = f[q].y * ((T)1 + bbv / (T)2 * f[q].y / rb);
+          const T pa = pr[q].x * ((T)1 + ba / (T)2 * pr[q].x / ra);
+          const T pb = pr[q].y * ((T)1 + bbv / (T)2 * pr[q].y / rb);
+          g[q].x = (ea - pa) * a.inv_dt;
+          g[q].y = (eb - pb) * a.inv_dt;
+          f[q].x = ea;
+          f[q].y = eb;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+          g[q].x = (f[q].x - pr[q].x) * a.inv_dt;
+          g[q].y = (f[q].y - pr[q].y) * a.inv_dt;
+        }
+      }
+    } else if (a.nonlinear) {
+      Cx<T> bb[E];
+      if (a.bona) L.load(a.bona, bb);
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const T ba = a.bona ? bb[q].x : a.bona_s, bbv = a.bona ? bb[q].y : a.bona_s;
+        const T ra = a.rho0 ? r0[q].x : a.rho0_s, rb = a.rho0 ? r0[q].y : a.rho0_s;
+        f[q].x = f[q].x * ((T)1 + ba / (T)2 * f[q].x / ra);
+        f[q].y = f[q].y * ((T)1 + bbv / (T)2 * f[q].y / rb);
+      }
+    }
+    if (a.c0sq) {
+      Cx<T> cc[E];
+      L.load(a.c0sq, cc);
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        f[q].x *= cc[q].x;
+        f[q].y *= cc[q].y;
+      }
+    }
+    // r2c of the pair: Z = FFT(f_a + i f_b); A_k = (Z_k + conj Z_{N-k})/2,
+    // B_k = (Z_k - conj Z_{N-k})/(2i), k = 0..N/2.
+    constexpr int NH = N / 2 + 1;
+#pragma unroll 1
+    for (int s = 0; s < 2; ++s) {
+      if (s == 1 && a.S2 == nullptr) break;
+      Cx<T>* Sdst = s == 0 ? a.S1 : a.S2;
+      Cx<T> z[E];
+#pragma unroll
+      for (int q = 0; q < E; ++q) z[q] = s == 0 ? f[q] : g[q];
+      line_fft<T, N, false>(z, lds, a.tw, L.t);
+#pragma unroll
+      for (int q = 0; q < E; ++q) lds[LdsLayout<N>::at(L.n(q))] = z[q];
+      __syncthreads();
+      const long long pi = (long long)blockIdx.x * lpw + L.c;
+      const long long oa = 2 * pi * NH, ob = (2 * pi + 1) * NH;
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const int k = L.n(q);
+        if (k < NH) {
+          const int km = (k == 0) ? 0 : N - k;
+          const Cx<T> zk = z[q];
+          const Cx<T> zm = lds[LdsLayout<N>::at(km)];
+          const Cx<T> A = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
+          const Cx<T> B = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
+          if (L.va) Sdst[oa + k] = A;
+          if (L.vb) Sdst[ob + k] = B;
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Spectral passes on the half spectrum (complex lines along y or z)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SpecArgs {
+  Cx<T>* S1;
+  Cx<T>* S2;          // second spectrum (:stokes) or null
+  const Cx<T>* tw;
+  const T* M1;        // octant multiplier tables or null
+  const T* M2;
+  int NH;             // Nx/2+1 (lines along kx)
+  long long Sa;       // stride along the line (complex elements)
+  long long So;       // outer stride
+  int x_blocks;
+  int mode;           // 0 fwd, 1 inv, 2 fwd*M*inv
+  // octant table addressing: index = kx + MH1*(|i1| + MH2*|i2|)
+  int axis;           // 1 = line along y, 2 = line along z
+  int Ny, Nz;
+  int MH1, MH2;       // Ny/2+1, Nz/2+1 (octant extents)
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_spec_pass(SpecArgs<T> a) {
+  constexpr int TPL = Plan<N>::TPL;
+  constexpr int E = Plan<N>::E;
+  constexpr int P = LdsLayout<N>::PITCH;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
+  const int lpw = blockDim.x / TPL;
+  const int c = threadIdx.x % lpw, t = threadIdx.x / lpw;
+  const int xb = blockIdx.x % a.x_blocks;
+  const int o = blockIdx.x / a.x_blocks;
+  const int kx = xb * lpw + c;
+  const bool valid = kx < a.NH;
+  Cx<T>* lds = lds_all + c * P;
+  const long long base = (long long)kx + a.So * o;
+
+  Cx<T> v[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = valid ? a.S1[base + a.Sa * (t + TPL * q)] : Cx<T>{0, 0};
+  if (a.mode == 1) {
+    line_fft<T, N, true>(v, lds, a.tw, t);
+  } else {
+    line_fft<T, N, false>(v, lds, a.tw, t);
+    if (a.mode == 2) {
+      Cx<T> w[E];
+      if (a.S2) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) w[q] = valid ? a.S2[base + a.Sa * (t + TPL * q)] : Cx<T>{0, 0};
+        line_fft<T, N, false>(w, lds, a.tw, t);
+      }
+      // octant index of this line's other coordinate
+      int io;  // |index| of the outer coordinate
+      if (a.axis == 1) {
+        io = o;  // outer = z
+        io = (a.Nz > 1 && io > a.Nz - io) ? a.Nz - io : io;
+      } else {
+        io = o;  // outer = y
+        io = (io > a.Ny - io) ? a.Ny - io : io;
+      }
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        int ik = t + TPL * q;
+        ik = (ik > N - ik) ? N - ik : ik;
+        long long mi;
+        if (a.axis == 1)
+          mi = (long long)kx + (long long)a.MH1 * (ik + (long long)a.MH2 * io);
+        else
+          mi = (long long)kx + (long long)a.MH1 * (io + (long long)a.MH2 * ik);
+        const T m1 = valid ? a.M1[mi] : (T)0;
+        Cx<T> r = cscale(v[q], m1);
+        if (a.S2) {
+          const T m2 = valid ? a.M2[mi] : (T)0;
+          r = cadd(r, cscale(w[q], m2));
+        }
+        v[q] = r;
+      }
+      line_fft<T, N, true>(v, lds, a.tw, t);
+    } else if (a.S2) {
+      // forward pass of the second spectrum too
+      if (valid) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) a.S1[base + a.Sa * (t + TPL * q)] = v[q];
+      }
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = valid ? a.S2[base + a.Sa * (t + TPL * q)] : Cx<T>{0, 0};
+      line_fft<T, N, false>(v, lds, a.tw, t);
+      if (valid) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) a.S2[base + a.Sa * (t + TPL * q)] = v[q];
+      }
+      return;
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int q = 0; q < E; ++q) a.S1[base + a.Sa * (t + TPL * q)] = v[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Complex-to-real pass along x: p = irfft(S) (1/N folded into M); 1-D grids
+// also apply the multiplier here.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct C2RArgs {
+  const Cx<T>* S1;
+  const Cx<T>* S2;   // 1-D :stokes second spectrum
+  const T* M1;       // 1-D only (length Nx/2+1) or null
+  const T* M2;
+  const Cx<T>* tw;
+  T* p;
+  int n_lines;
+};
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_c2r_x(C2RArgs<T> a) {
+  constexpr int TPL = Plan<N>::TPL;
+  constexpr int E = Plan<N>::E;
+  constexpr int P = LdsLayout<N>::PITCH;
+  constexpr int NH = N / 2 + 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
+  const int lpw = blockDim.x / TPL;
+  const int t = threadIdx.x % TPL, c = threadIdx.x / TPL;
+  const long long pi = (long long)blockIdx.x * lpw + c;
+  const long long la = 2 * pi, lb = 2 * pi + 1;
+  const bool va = la < a.n_lines, vb = lb < a.n_lines;
+  Cx<T>* lds = lds_all + c * P;
+  Cx<T> z[E];
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int k = t + TPL * q;
+    const bool mir = k >= NH;
+    const int kk = mir ? N - k : k;
+    Cx<T> A = va ? a.S1[la * NH + kk] : Cx<T>{0, 0};
+    Cx<T> B = vb ? a.S1[lb * NH + kk] : Cx<T>{0, 0};
+    if (a.M1) {
+      const T m1 = a.M1[kk];
+      A = cscale(A, m1);
+      B = cscale(B, m1);
+      if (a.S2) {
+        const T m2 = a.M2[kk];
+        const Cx<T> A2 = va ? a.S2[la * NH + kk] : Cx<T>{0, 0};
+        const Cx<T> B2 = vb ? a.S2[lb * NH + kk] : Cx<T>{0, 0};
+        A = cadd(A, cscale(A2, m2));
+        B = cadd(B, cscale(B2, m2));
+      }
+    }
+    // c2r semantics: imaginary parts of the DC and (even-N) Nyquist bins are ignored
+    if (kk == 0 || 2 * kk == N) {
+      A.y = 0;
+      B.y = 0;
+    }
+    if (mir) {
+      A.y = -A.y;
+      B.y = -B.y;
+    }
+    z[q] = Cx<T>{A.x - B.y, A.y + B.x};
+  }
+  line_fft<T, N, true>(z, lds, a.tw, t);
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int n = t + TPL * q;
+    if (va) a.p[la * N + n] = z[q].x;
+    if (vb) a.p[lb * N + n] = z[q].y;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Sensor recording (record_sensor_data!, first_order_steps.jl:728-794)
+// ---------------------------------------------------------------------------
+enum : unsigned {
+  REC_P = 1u << 0,
+  REC_P_MAX = 1u << 1,
+  REC_P_MIN = 1u << 2,
+  REC_P_RMS = 1u << 3,
+  REC_P_FINAL = 1u << 4,
+  REC_UX = 1u << 5,
+  REC_UY = 1u << 6,
+  REC_UZ = 1u << 7,
+  REC_U_MAX = 1u << 8,
+  REC_U_RMS = 1u << 9,
+  REC_U_FINAL = 1u << 10,
+  REC_I_AVG = 1u << 11,
+  REC_I_MAX = 1u << 12,
+};
+
+template <typename T>
+struct RecArgs {
+  const T* p;
+  const T* u[3];
+  int ndim;
+  const long long* idx;
+  int n;
+  int t;       // 1-based
+  int Nt;
+  unsigned rec;
+  T* p_ts;     // [t][j]
+  T* u_ts[3];  // [t][j]
+  T* p_max;
+  T* p_min;
+  double* p_rms;
+  T* p_final;
+  double* u_max;
+  double* u_rms;
+  double* u_final;
+  double* I_avg;
+  double* I_max;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_record(RecArgs<T> a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  const long long i = a.idx[j];
+  const T pv = a.p[i];
+  const long long o = (long long)(a.t - 1) * a.n + j;
+  if (a.rec & REC_P) a.p_ts[o] = pv;
+  if (a.rec & REC_P_MAX) a.p_max[j] = a.p_max[j] > pv ? a.p_max[j] : pv;
+  if (a.rec & REC_P_MIN) a.p_min[j] = a.p_min[j] < pv ? a.p_min[j] : pv;
+  if (a.rec & REC_P_RMS) a.p_rms[j] += (double)(pv * pv);
+  if ((a.rec & REC_P_FINAL) && a.t == a.Nt) a.p_final[j] = pv;
+  T uv[3] = {0, 0, 0};
+  for (int d = 0; d < a.ndim; ++d) uv[d] = a.u[d][i];
+  if (a.rec & REC_UX) a.u_ts[0][o] = uv[0];
+  if ((a.rec & REC_UY) && a.ndim > 1) a.u_ts[1][o] = uv[1];
+  if ((a.rec & REC_UZ) && a.ndim > 2) a.u_ts[2][o] = uv[2];
+  if (a.rec & (REC_U_MAX | REC_U_RMS | REC_U_FINAL)) {
+    double m2 = 0.0;
+    for (int d = 0; d < a.ndim; ++d) m2 += (double)(uv[d] * uv[d]);
+    const double mag = sqrt(m2);
+    if (a.rec & REC_U_MAX) a.u_max[j] = a.u_max[j] > mag ? a.u_max[j] : mag;
+    if (a.rec & REC_U_RMS) a.u_rms[j] += mag * mag;
+    if ((a.rec & REC_U_FINAL) && a.t == a.Nt) a.u_final[j] = mag;
+  }
+  if (a.rec & (REC_I_AVG | REC_I_MAX)) {
+    double s = 0.0;
+    for (int d = 0; d < a.ndim; ++d) {
+      const double pu = (double)pv * (double)uv[d];
+      s += pu * pu;
+    }
+    const double inst = sqrt(s);
+    if (a.rec & REC_I_AVG) a.I_avg[j] += inst / a.Nt;
+    if (a.rec & REC_I_MAX) a.I_max[j] = a.I_max[j] > inst ? a.I_max[j] : inst;
+  }
+}
+
+// Time-reversal Dirichlet injection before a step (first_order.jl:678-689):
+// rho_d[idx_j] = data[j, Nt - t + 1] * scale_j for every component d.
+template <typename T>
+__global__ void __launch_bounds__(256) k_tr_inject(T* r0, T* r1, T* r2, const long long* idx, const T* vals,
+                                                   int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const long long i = idx[j];
+  const T v = vals[j];
+  r0[i] = v;
+  if (r1) r1[i] = v;
+  if (r2) r2[i] = v;
+}
+
+// rho_d = p0 / (ndim c0^2) for every component; p = p0
+// (first_order_steps.jl:113-118, 161-167).
+template <typename T>
+__global__ void __launch_bounds__(256) k_init_rho(const T* p0, const T* c0sq, T c0sq_s, T ndim, T* p, T* r0, T* r1,
+                                                  T* r2, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T v = p0[i];
+  const T r = v / (ndim * (c0sq ? c0sq[i] : c0sq_s));
+  p[i] = v;
+  r0[i] = r;
+  if (r1) r1[i] = r;
+  if (r2) r2[i] = r;
+}
+
+// p_rms = sqrt(sum p^2 / Nt) (finalize_sensor_data!, first_order_steps.jl:818-825)
+template <typename D>
+__global__ void __launch_bounds__(256) k_finalize_rms(D* a, long long n, D nt) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = sqrt(a[i] / nt);
+}
+
+}  // namespace kw

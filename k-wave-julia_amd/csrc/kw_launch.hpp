@@ -1,0 +1,34 @@
+// Kernel launchers, one translation unit per precision and kernel family so
+// the many (precision x line length) instantiations build in parallel.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kw_kernels.hpp"
+
+// Supported line-FFT lengths (kw_fft.hpp Plan<N>).
+#define KW_SIZES(X) \
+  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(2048) X(4096)
+
+namespace kw {
+
+bool size_supported(long long n);
+int plan_tpl(int n);
+int plan_pitch(int n);
+
+// Return hipSuccess or the launch error; unsupported lengths give hipErrorInvalidValue.
+template <typename T>
+hipError_t launch_dir(int n, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const PassArgs<T>& a);
+template <typename T>
+hipError_t launch_spec(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<T>& a);
+template <typename T>
+hipError_t launch_c2r(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const C2RArgs<T>& a);
+template <typename T>
+hipError_t launch_record(unsigned nb, hipStream_t s, const RecArgs<T>& a);
+template <typename T>
+hipError_t launch_tr_inject(unsigned nb, hipStream_t s, T* r0, T* r1, T* r2, const long long* idx, const T* v, int n);
+template <typename T>
+hipError_t launch_init_rho(unsigned nb, hipStream_t s, const T* p0, const T* c0sq, T c0sq_s, T ndim, T* p, T* r0, T* r1,
+                           T* r2, long long n);
+hipError_t launch_finalize_rms(unsigned nb, hipStream_t s, double* a, long long n, double nt);
+
+}  // namespace kw

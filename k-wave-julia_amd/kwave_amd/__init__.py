@@ -1,0 +1,24 @@
+"""kwave_amd -- MI355X-native drop-in for KWave.jl's kspace_first_order.
+
+Host-side mirror of the reference API (KWave.jl/src/{grid,medium,source,
+sensor}.jl and solver/first_order.jl) over the C ABI of libkwave_hip.so
+(include/kwave_hip.h).  The time loop runs only on the GPU; if the shared
+library or the GPU is missing, the solver raises.
+"""
+from .model import (AbstractKWaveGrid, Additive, AdditiveNoCorrection, Dirichlet, KWaveGrid, KWaveGrid1D,
+                    KWaveGrid2D, KWaveGrid3D, KWaveMedium, KWaveSensor, KWaveSource, SimulationOutput,
+                    SourceMode, VALID_RECORD_FIELDS, has_p0, has_pressure_source, has_velocity_source,
+                    is_homogeneous, is_lossless, is_nonlinear, is_time_reversal, make_time,
+                    validate_record_fields)
+from .builders import make_ball, make_bowl, make_circle, make_disc, tone_burst
+from .solver import Simulation, build_desc, kspace_first_order
+from ._lib import KWaveHIPError, LIB_PATH
+
+__all__ = [
+    "AbstractKWaveGrid", "KWaveGrid", "KWaveGrid1D", "KWaveGrid2D", "KWaveGrid3D", "make_time",
+    "KWaveMedium", "KWaveSource", "KWaveSensor", "SimulationOutput", "SourceMode", "Dirichlet",
+    "Additive", "AdditiveNoCorrection", "VALID_RECORD_FIELDS", "validate_record_fields",
+    "is_homogeneous", "is_lossless", "is_nonlinear", "is_time_reversal", "has_p0",
+    "has_pressure_source", "has_velocity_source", "kspace_first_order", "Simulation", "build_desc",
+    "tone_burst", "make_disc", "make_circle", "make_ball", "make_bowl", "KWaveHIPError", "LIB_PATH",
+]

@@ -1,0 +1,154 @@
+"""ctypes binding of libkwave_hip.so (include/kwave_hip.h).
+
+The library is loaded from this package folder.  There is no fallback: if
+the shared object is missing or no GPU is visible, calls raise
+``KWaveHIPError`` -- the product path never computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkwave_hip.so")
+
+KW_OK = 0
+KW_ERR_INVALID = -1
+KW_ERR_UNSUPPORTED = -2
+KW_ERR_HIP = -3
+KW_ERR_OOM = -4
+KW_ERR_STATE = -5
+
+REC_BITS = {
+    "p": 1 << 0, "p_max": 1 << 1, "p_min": 1 << 2, "p_rms": 1 << 3, "p_final": 1 << 4,
+    "ux": 1 << 5, "uy": 1 << 6, "uz": 1 << 7, "u_max": 1 << 8, "u_rms": 1 << 9,
+    "u_final": 1 << 10, "I_avg": 1 << 11, "I_max": 1 << 12,
+}
+FIELD = {"p": 0, "ux": 1, "uy": 2, "uz": 3, "rhox": 4, "rhoy": 5, "rhoz": 6}
+ALPHA_MODE = {"no_absorption": 0, "no_dispersion": 1, "stokes": 2}
+
+# every symbol include/kwave_hip.h declares
+EXPORTS = (
+    "kw_abi_version", "kw_last_error", "kw_device_count", "kw_ctx_create", "kw_ctx_destroy",
+    "kw_fft_length_supported", "kw_sim_create", "kw_sim_run", "kw_sim_synchronize",
+    "kw_sim_steps_done", "kw_sim_get_field", "kw_sim_sensor_shape", "kw_sim_get_sensor",
+    "kw_sim_profile", "kw_sim_destroy",
+)
+
+
+class KWaveHIPError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(msg)
+        self.code = code
+
+
+class SimDesc(C.Structure):
+    """Mirror of kw_sim_desc (include/kwave_hip.h)."""
+    _fields_ = [
+        ("ndim", C.c_int32),
+        ("N", C.c_int64 * 3),
+        ("d", C.c_double * 3),
+        ("dt", C.c_double),
+        ("Nt", C.c_int64),
+        ("precision", C.c_int32),
+        ("map_storage", C.c_int32),
+        ("pml_size", C.c_int32 * 3),
+        ("pml_alpha", C.c_double * 3),
+        ("c_ref", C.c_double),
+        ("c0", C.POINTER(C.c_double)),
+        ("c0_scalar", C.c_double),
+        ("rho0", C.POINTER(C.c_double)),
+        ("rho0_scalar", C.c_double),
+        ("nonlinear", C.c_int32),
+        ("bona", C.POINTER(C.c_double)),
+        ("bona_scalar", C.c_double),
+        ("absorbing", C.c_int32),
+        ("alpha_mode", C.c_int32),
+        ("alpha_coeff", C.c_double),
+        ("alpha_power", C.c_double),
+        ("p0", C.POINTER(C.c_double)),
+        ("smooth_p0", C.c_int32),
+        ("n_psrc", C.c_int64),
+        ("psrc_index", C.POINTER(C.c_int64)),
+        ("psrc_signal", C.POINTER(C.c_double)),
+        ("psrc_ncols", C.c_int64),
+        ("p_mode", C.c_int32),
+        ("n_usrc", C.c_int64),
+        ("usrc_index", C.POINTER(C.c_int64)),
+        ("usrc_signal", C.POINTER(C.c_double) * 3),
+        ("usrc_ncols", C.c_int64 * 3),
+        ("u_mode", C.c_int32),
+        ("n_sensor", C.c_int64),
+        ("sensor_index", C.POINTER(C.c_int64)),
+        ("record", C.c_uint32),
+        ("tr_data", C.POINTER(C.c_double)),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load the shared library once (raises if it is missing)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise KWaveHIPError(KW_ERR_STATE, f"{LIB_PATH} is missing: build it with "
+                                "`python k-wave-julia_amd/csrc/build.py` (hipcc, gfx950)")
+        lib = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        sig = {
+            "kw_abi_version": ([], C.c_int32),
+            "kw_last_error": ([], C.c_char_p),
+            "kw_device_count": ([C.POINTER(C.c_int32)], C.c_int32),
+            "kw_ctx_create": ([C.c_int32, C.POINTER(P)], C.c_int32),
+            "kw_ctx_destroy": ([P], C.c_int32),
+            "kw_fft_length_supported": ([C.c_int64], C.c_int32),
+            "kw_sim_create": ([P, C.POINTER(SimDesc), C.POINTER(P)], C.c_int32),
+            "kw_sim_run": ([P, C.c_int64], C.c_int32),
+            "kw_sim_synchronize": ([P], C.c_int32),
+            "kw_sim_steps_done": ([P, C.POINTER(C.c_int64)], C.c_int32),
+            "kw_sim_get_field": ([P, C.c_int32, C.POINTER(C.c_double)], C.c_int32),
+            "kw_sim_sensor_shape": ([P, C.c_uint32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int32),
+            "kw_sim_get_sensor": ([P, C.c_uint32, C.POINTER(C.c_double)], C.c_int32),
+            "kw_sim_profile": ([P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_char_p),
+                                C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int32),
+            "kw_sim_destroy": ([P], C.c_int32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = res
+        if lib.kw_abi_version() != 1:
+            raise KWaveHIPError(KW_ERR_STATE, "libkwave_hip ABI version mismatch")
+        _lib = lib
+        return lib
+
+
+def check(code: int) -> None:
+    if code != KW_OK:
+        msg = _lib.kw_last_error().decode("utf-8", "replace") if _lib is not None else ""
+        raise KWaveHIPError(code, msg or f"libkwave_hip error {code}")
+
+
+_ctx = {}
+
+
+def context(device: int = 0):
+    """Process-wide device context for `device` (created on first use)."""
+    lib = load()
+    with _lock:
+        if device in _ctx:
+            return _ctx[device]
+        n = C.c_int32(0)
+        check(lib.kw_device_count(C.byref(n)))
+        if n.value == 0:
+            raise KWaveHIPError(KW_ERR_HIP, "no HIP device visible")
+        h = C.c_void_p()
+        check(lib.kw_ctx_create(int(device), C.byref(h)))
+        _ctx[device] = h
+        return h

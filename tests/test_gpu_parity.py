@@ -1,0 +1,107 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
+
+FP64 kernels must match the oracle to <= 1e-10 relative L2 and FP32 kernels
+to <= 1e-4, on every recorded array.
+"""
+import copy
+import sys
+
+import numpy as np
+import pytest
+
+import kwave_amd as K
+from oracle import kwave_oracle as O
+
+from cases import rel_l2, scenario
+
+pytestmark = pytest.mark.gpu
+
+TOL = {np.float64: 1e-10, np.float32: 1e-4}
+
+CASES = [
+    ("psrc", (64, 48)),
+    ("p0_psrc", (48, 40)),
+    ("nodisp_psrc", (64, 64)),
+    ("stokes_p0_psrc", (40, 48)),
+    ("nonlin_psrc", (64, 32)),
+    ("hetc_hetr_psrc", (48, 64)),
+    ("hetc_absorb_nonlin_psrc", (64, 48)),
+    ("hetc_hetr_nonlin_hetb_p0_urec", (32, 32)),
+    ("stokes_nonlin_psrc", (32, 48)),
+    ("pdir_usrc_urec", (48, 48)),
+    ("udir_psrc_short", (32, 64)),
+    ("usrc_only_ux_hetr", (64, 32)),
+    ("pmlhard_p0_psrc", (64, 64)),
+    ("psrc", (96,)),
+    ("p0_stokes_psrc", (128,)),
+    ("hetc_hetr_nonlin_psrc_urec", (64,)),
+    ("usrc_urec", (48,)),
+    ("psrc", (24, 16, 32)),
+    ("hetc_absorb_nonlin_psrc", (32, 24, 16)),
+    ("stokes_p0_nonlin_psrc", (16, 32, 24)),
+    ("hetc_hetr_usrc_urec_p0", (16, 16, 32)),
+    ("pdir_psrc_irec", (24, 24, 24)),
+]
+
+
+def _compare(out, ref, tol, label):
+    bad = []
+    for k in out.keys():
+        if k.startswith("_"):
+            continue
+        e = rel_l2(out[k], ref[k])
+        print(f"    {label} {k:8s} rel-L2 {e:.3e}", flush=True)
+        if not e <= tol:
+            bad.append((k, e))
+    assert not bad, f"{label}: {bad} exceed {tol}"
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("name,shape", CASES, ids=[f"{n}-{'x'.join(map(str, s))}" for n, s in CASES])
+def test_parity_vs_oracle(name, shape, dtype):
+    if "irec" in name and dtype == np.float32:
+        pytest.skip("I_avg/I_max are Float64-only in the reference (first_order_steps.jl:804)")
+    print(f"\n[case] {name} {shape} {dtype.__name__}", flush=True)
+    kg, med, src, sen, kw = scenario(name, shape)
+    ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
+    out = K.kspace_first_order(kg, med, src, sen, data_cast=dtype, **kw)
+    assert set(out.keys()) == {k for k in ref if not k.startswith("_")}
+    _compare(out, ref, TOL[dtype], f"{name}/{dtype.__name__}")
+
+
+SIZES = [8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 2048, 4096]
+
+
+@pytest.mark.parametrize("n", SIZES)
+def test_every_line_length_1d(n):
+    """Each compiled line-FFT plan, as the x axis of a 1-D run (FP64)."""
+    print(f"\n[case] 1d size {n}", flush=True)
+    kg, med, src, sen, kw = scenario("p0_psrc", (n,), nt=20)
+    ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
+    out = K.kspace_first_order(kg, med, src, sen, data_cast=np.float64, **kw)
+    _compare(out, ref, 1e-10, f"1d-{n}")
+
+
+@pytest.mark.parametrize("shape", [(8, 384), (768, 12), (12, 96, 8), (8, 8, 192)])
+def test_strided_line_lengths(shape):
+    """Plans used on the strided (y/z) axes and the spectral passes (FP64)."""
+    print(f"\n[case] strided {shape}", flush=True)
+    kg, med, src, sen, kw = scenario("hetc_nonlin_psrc", shape, nt=15)
+    ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
+    out = K.kspace_first_order(kg, med, src, sen, data_cast=np.float64, **kw)
+    _compare(out, ref, 1e-10, f"strided-{shape}")
+
+
+def test_time_reversal_2d():
+    """TR reconstruction (first_order.jl:426-460, 491-494) vs the oracle."""
+    kg, med, src, sen, kw = scenario("p0", (48, 48), nt=50)
+    fwd = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
+    tr_sensor = K.KWaveSensor(mask=sen.mask, time_reversal_boundary_data=fwd["p"], record=["p_final"])
+    kg2 = copy.deepcopy(kg)
+    ref = O.kspace_first_order(copy.deepcopy(kg2), med, K.KWaveSource(), tr_sensor, **kw)
+    for dt, tol in ((np.float64, 1e-10), (np.float32, 1e-4)):
+        out = K.kspace_first_order(copy.deepcopy(kg2), med, K.KWaveSource(), tr_sensor, data_cast=dt, **kw)
+        e = rel_l2(out["p_final"], ref["p_final"])
+        print(f"    tr {dt.__name__} p_final rel-L2 {e:.3e}", flush=True)
+        assert e <= tol
+        assert out["p_final"].shape == (48 * 48,)

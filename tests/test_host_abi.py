@@ -1,0 +1,115 @@
+"""Host-side checks of the C ABI and the Python mirror (no GPU compute)."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import kwave_amd as K
+from kwave_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "kwave_hip.h")
+
+
+def _declared_functions():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\w[\w\s\*]*?\b(kw_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_header_declares_the_binding_table():
+    assert set(_declared_functions()) == set(_lib.EXPORTS)
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libkwave_hip.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(_lib.LIB_PATH)
+    for name in _declared_functions():
+        assert hasattr(lib, name), name
+    lib = _lib.load()
+    assert lib.kw_abi_version() == 1
+    for n in (8, 24, 128, 256, 384, 512, 768, 1024, 4096):
+        assert lib.kw_fft_length_supported(n) == 1
+    for n in (7, 100, 1000, 8192):
+        assert lib.kw_fft_length_supported(n) == 0
+
+
+def test_ctypes_descriptor_layout_matches_header():
+    """Compile the header with gcc and compare sizeof/offsetof with ctypes."""
+    fields = [f[0] for f in _lib.SimDesc._fields_]
+    src = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"', "int main(void){",
+           'printf("size %zu\\n", sizeof(kw_sim_desc));']
+    for f in fields:
+        src.append(f'printf("{f} %zu\\n", offsetof(kw_sim_desc, {f}));')
+    src.append("return 0;}")
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "t.c")
+        exe = os.path.join(d, "t")
+        open(c, "w").write("\n".join(src))
+        subprocess.run(["gcc", "-std=c99", c, "-o", exe], check=True)
+        out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout.split("\n")
+    vals = dict(l.split() for l in out if l)
+    assert int(vals["size"]) == C.sizeof(_lib.SimDesc)
+    for f in fields:
+        assert int(vals[f]) == getattr(_lib.SimDesc, f).offset, f
+
+
+def _small():
+    kg = K.KWaveGrid(32, 1e-4, 32, 1e-4)
+    med = K.KWaveMedium(sound_speed=1500.0, density=1000.0)
+    src = K.KWaveSource(p0=np.zeros((32, 32)))
+    m = np.zeros((32, 32), bool)
+    m[3, 4] = True
+    return kg, med, src, m
+
+
+def test_validation_errors_match_reference():
+    kg, med, src, m = _small()
+    with pytest.raises(ValueError, match="Time array not set"):                   # first_order.jl:311-313
+        K.kspace_first_order(kg, med, src, K.KWaveSensor(mask=m))
+    K.make_time(kg, 1500.0)
+    with pytest.raises(ValueError, match="Invalid sensor record field"):          # sensor.jl:54-56
+        K.kspace_first_order(kg, med, src, K.KWaveSensor(mask=m, record=["p", "pressure"]))
+    with pytest.raises(TypeError):                                                # first_order.jl:294
+        K.build_desc(kg, med, src, K.KWaveSensor(mask=m), pml_alpha=2)
+    with pytest.raises(TypeError):
+        K.build_desc(kg, med, src, K.KWaveSensor(mask=m), data_cast=np.float16)
+
+
+def test_descriptor_contents():
+    kg, med, src, m = _small()
+    K.make_time(kg, 1500.0)
+    c0 = np.full((32, 32), 1500.0)
+    c0[5:, :] = 1600.0
+    med = K.KWaveMedium(sound_speed=c0, density=1000.0, alpha_coeff=0.75, alpha_power=1.5, alpha_mode="stokes",
+                        BonA=5.0)
+    pm = np.zeros((32, 32), bool)
+    pm[4, 7] = pm[9, 2] = True
+    src = K.KWaveSource(p_mask=pm, p=np.ones((2, 10)))
+    d, keep, info = K.build_desc(kg, med, src, K.KWaveSensor(mask=m, record=["p", "p_rms"]),
+                                 pml_size=(10, 12), pml_alpha=(2.0, 3.0), data_cast=np.float32)
+    assert d.ndim == 2 and tuple(d.N) == (32, 32, 1) and d.precision == 4
+    assert tuple(d.pml_size)[:2] == (10, 12) and tuple(d.pml_alpha)[:2] == (2.0, 3.0)
+    assert d.c_ref == pytest.approx(np.mean(c0)) and d.absorbing == 1 and d.alpha_mode == 2
+    assert d.nonlinear == 1 and d.bona_scalar == 5.0
+    assert d.n_psrc == 2 and [d.psrc_index[i] for i in range(2)] == [9 + 32 * 2, 4 + 32 * 7]  # findall order
+    assert d.record == _lib.REC_BITS["p"] | _lib.REC_BITS["p_rms"] and d.n_sensor == 1
+    assert d.sensor_index[0] == 3 + 32 * 4
+
+
+def test_product_fails_loudly_without_library(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    kg, med, src, m = _small()
+    K.make_time(kg, 1500.0)
+    with pytest.raises(_lib.KWaveHIPError, match="missing"):
+        K.kspace_first_order(kg, med, src, K.KWaveSensor(mask=m), smooth_p0=False)
+
+
+def test_simulation_output_mirror():
+    out = K.SimulationOutput({"p": np.zeros((2, 3)), "p_max": np.zeros(2)})
+    assert out[":p"].shape == (2, 3) and "p_max" in out and out.haskey("p") and set(out.keys()) == {"p", "p_max"}
+    assert repr(out) == "SimulationOutput with fields: p, p_max"

@@ -1,0 +1,363 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each check is chosen so that a plausible misreading of the reference (a
+dropped term, a wrong sign, a wrong shift direction, a wrong index order)
+fails it: the reference's own test assertions (test_grid.jl, test_pml.jl,
+test_fft.jl, test_signal.jl, test_geometry.jl, test_absorption.jl,
+test_solver_*.jl, test_time_reversal.jl), closed forms, the exact per-mode
+recurrence (golden fixtures) and brute force on tiny grids.
+"""
+import copy
+import math
+import os
+
+import numpy as np
+import pytest
+
+import kwave_amd as K
+from oracle import closed_form, kwave_oracle as O
+
+from cases import rel_l2, scenario
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------- grid (test_grid.jl)
+def test_grid_vectors_and_time():
+    g = O.Grid((64,), (1e-4,))
+    assert math.isclose(O.make_spatial_vec(64, 1e-4)[0], -(64 - 1) / 2 * 1e-4)
+    assert g.k_vecs[0][0] == 0.0 and np.all(g.k >= 0)
+    sp, sn = g.shifts[0]
+    assert np.allclose(np.abs(sp), 1) and np.allclose(np.abs(sn), 1)
+    kv = O.make_wavenumber_vec(8, 1.0)
+    assert kv.size == 8 and kv[0] == 0.0
+    # closed form: [0,1,2,3,-4,-3,-2,-1]*2pi/8
+    assert np.allclose(kv, np.array([0, 1, 2, 3, -4, -3, -2, -1]) * 2 * math.pi / 8, atol=0, rtol=1e-15)
+    kv7 = O.make_wavenumber_vec(7, 1.0)
+    assert np.allclose(kv7, np.array([0, 1, 2, 3, -3, -2, -1]) * 2 * math.pi / 7, atol=0, rtol=1e-15)
+    g2 = O.Grid((64, 128), (1e-4, 2e-4))
+    for j in range(5):
+        for i in range(5):
+            assert math.isclose(g2.k[i, j], math.hypot(g2.k_vecs[0][i], g2.k_vecs[1][j]))
+    dt, nt = O.make_time((128, 128), (1e-4, 1e-4), 1500.0)
+    assert math.isclose(dt, 0.3 * 1e-4 / 1500.0)
+    dt2, nt2 = O.make_time((128, 128), (1e-4, 1e-4), 1500.0, t_end=1e-5)
+    assert nt2 == math.ceil(1e-5 / dt2) + 1
+    # configuration 1: default t_end = 2*diag/c -> Nt = 1208
+    assert nt == 1208
+
+
+def test_product_grid_matches_oracle():
+    kg = K.KWaveGrid(64, 1e-4, 128, 2e-4)
+    g = O.Grid((64, 128), (1e-4, 2e-4))
+    assert np.array_equal(kg.kx_vec, g.k_vecs[0]) and np.array_equal(kg.ky_vec, g.k_vecs[1])
+    assert np.array_equal(kg.k, g.k)
+    assert kg.k_max() == pytest.approx(math.pi / 2e-4)
+    dt, nt = K.make_time(kg, 1500.0)
+    assert (kg.dt, kg.Nt) == O.make_time((64, 128), (1e-4, 2e-4), 1500.0)
+
+
+# ---------------------------------------------------------------- PML (test_pml.jl)
+def test_pml_reference_assertions_and_closed_form():
+    N, S = 64, 10
+    p = O.get_pml(N, 1e-4, S, 2.0, 1e-8)
+    assert np.all(p[S:N - S] == 1.0) and p[0] < 1 and p[-1] < 1 and np.all((p > 0) & (p <= 1))
+    assert np.allclose(p[:S], p[::-1][:S])
+    assert np.all(np.diff(p[:S]) >= 0)
+    ps = O.get_pml(N, 1e-4, S, 2.0, 1e-8, staggered=True)
+    assert not np.array_equal(p, ps) and np.all(ps[11:54] == 1.0)
+    assert np.all(O.get_pml(N, 1e-4, 0, 2.0, 1e-8) == 1.0)
+    # closed form: outermost point pos=1 (exp(-alpha dt)), staggered pos=(S-0.5)/S
+    alpha, dt = 7.0e6, 2e-8
+    q = O.get_pml(N, 1e-4, S, alpha, dt)
+    qs = O.get_pml(N, 1e-4, S, alpha, dt, staggered=True)
+    assert q[0] == pytest.approx(math.exp(-alpha * dt), rel=1e-15)
+    assert q[S - 1] == pytest.approx(math.exp(-alpha * (1 / S) ** 4 * dt), rel=1e-15)
+    assert qs[0] == pytest.approx(math.exp(-alpha * ((S - 0.5) / S) ** 4 * dt), rel=1e-15)
+    assert qs[-1] == qs[0]
+
+
+# ---------------------------------------------------------------- absorption
+def test_db2neper_and_absorption_prefactors():
+    a = O.db2neper(0.75, 1.5)
+    # conversion.jl:34 with the reference's 1e-2 factor (5.48e-14)
+    assert a == pytest.approx(0.75 * 1e-2 / 8.685889638065035 * (2 * math.pi * 1e6) ** -1.5, rel=1e-14)
+    assert a == pytest.approx(5.48e-14, rel=2e-3)
+    assert O.db2neper(1.0, 1.0) > O.db2neper(1.0, 2.0)
+    med = K.KWaveMedium(sound_speed=1500.0, alpha_coeff=0.75, alpha_power=1.5, alpha_mode="stokes")
+    ab = O.precompute_absorption(med, np.array([0.0, 1.0, 4.0]), 1500.0)
+    assert ab["tau"] == pytest.approx(-2 * a * 1500.0 ** 0.5)
+    assert ab["eta"] == pytest.approx(2 * a * 1500.0 ** 0.5 * math.tan(0.75 * math.pi))
+    assert np.allclose(ab["nabla1"], ab["tau"] * np.array([0.0, 1.0, 8.0]))
+    assert O.precompute_absorption(K.KWaveMedium(sound_speed=1500.0, alpha_coeff=0.75), np.ones(2), 1.0) is None
+
+
+def test_kappa_closed_form():
+    k = np.array([0.0, 1000.0, 31415.9])
+    kap = O.compute_kappa(k, 1500.0, 2e-8)
+    x = 1500.0 * k * 2e-8 / 2
+    assert kap[0] == 1.0 and np.allclose(kap[1:], np.sin(x[1:]) / x[1:], rtol=1e-15)
+
+
+# ---------------------------------------------------------------- inputs (test_signal.jl, test_geometry.jl)
+def test_tone_burst_and_c1_golden():
+    s = O.tone_burst(40e6, 1e6, 5)
+    assert abs(s[0]) < 0.1 and abs(s[-1]) < 0.1 and 0.5 < np.max(np.abs(s)) <= 1 + 1e-10
+    so = O.tone_burst(40e6, 1e6, 5, signal_offset=100)
+    assert so.size == s.size + 100 and np.all(so[:100] == 0)
+    g = np.load(os.path.join(GOLD, "c1_inputs.npz"))
+    b = O.tone_burst(1 / float(g["dt"]), 1e6, 5)
+    assert b.size == 250 and np.allclose(b, g["tone_burst"], rtol=0, atol=1e-14)
+    assert np.array_equal(K.tone_burst(1 / float(g["dt"]), 1e6, 5), b)
+
+
+def test_masks_reference_assertions_and_brute_force():
+    disc = O.make_disc(64, 64, 32, 32, 10)
+    assert disc[31, 31] and not disc[0, 0] and abs(disc.sum() - math.pi * 100) / (math.pi * 100) < 0.1
+    circ = O.make_circle(64, 64, 32, 32, 15)
+    assert not circ[31, 31] and abs(circ.sum() - 2 * math.pi * 15) / (2 * math.pi * 15) < 0.3
+    ball = O.make_ball(32, 32, 32, 16, 16, 16, 8)
+    assert abs(ball.sum() - 4 / 3 * math.pi * 512) / (4 / 3 * math.pi * 512) < 0.15
+    g = np.load(os.path.join(GOLD, "c1_inputs.npz"))
+    d1 = O.make_disc(128, 128, 64, 64, 5)
+    assert np.array_equal(O._findall(d1), g["disc_idx"]) and d1.sum() == 81
+    assert O.make_circle(128, 128, 64, 64, 40).sum() == 228           # configuration 1 sensor
+    for f, args in ((O.make_disc, (40, 30, 17, 11, 6)), (O.make_circle, (40, 30, 17, 11, 9)),
+                    (O.make_ball, (20, 18, 16, 9, 8, 7, 5))):
+        assert np.array_equal(f(*args), getattr(K, f.__name__)(*args))
+    bo = O.make_bowl(40, 40, 40, (20, 20, 25), 15, 20, (20, 20, 5))
+    assert np.array_equal(bo, K.make_bowl(40, 40, 40, (20, 20, 25), 15, 20, (20, 20, 5)))
+    assert bo.sum() > 0 and not bo[:, :, 30:].any()
+
+
+# ---------------------------------------------------------------- spectral derivative (test_fft.jl)
+def test_spectral_gradient_analytic():
+    Nx, Ny = 64, 64
+    g = O.Grid((Nx, Ny), (1.0, 1.0))
+    x = O.make_spatial_vec(Nx, 1.0)
+    k0 = 2 * math.pi / Nx
+    f = np.sin(k0 * x)[:, None] * np.ones((1, Ny))
+    df = O.spectral_gradient(f, g.k_vecs[0], np.ones(Nx, dtype=complex), 0)
+    assert np.allclose(df[4:-4, Ny // 2], k0 * np.cos(k0 * x[4:-4]), atol=1e-10)
+    # staggered +: derivative sampled at x + h/2; staggered -: at x - h/2
+    for m in (1, 5, 17):
+        km = 2 * math.pi * m / Nx
+        fy = np.ones((Nx, 1)) * np.sin(km * x)[None, :]
+        dp = O.spectral_gradient(fy, g.k_vecs[1], g.shifts[1][0], 1)
+        dm = O.spectral_gradient(fy, g.k_vecs[1], g.shifts[1][1], 1)
+        assert np.allclose(dp[3, :], km * np.cos(km * (x + 0.5)), atol=1e-11)
+        assert np.allclose(dm[3, :], km * np.cos(km * (x - 0.5)), atol=1e-11)
+    # Nyquist: cos(pi x) shifted by h/2 has derivative -pi sin(pi (x+1/2))
+    fn = np.cos(math.pi * np.arange(Nx))[:, None] * np.ones((1, 3))
+    dn = O.spectral_gradient(fn, g.k_vecs[0], O.Grid((Nx, 3), (1.0, 1.0)).shifts[0][0], 0)
+    assert np.allclose(dn[:, 0], -math.pi * np.sin(math.pi * (np.arange(Nx) + 0.5)), atol=1e-11)
+
+
+def test_spectral_gradient_dense_dft_brute_force():
+    """Explicit DFT sums (no FFT library) on an odd and an even length."""
+    for N in (7, 10):
+        h = 0.3
+        k = O.make_wavenumber_vec(N, h)
+        n = np.arange(N)
+        F = np.exp(-2j * math.pi * np.outer(n, n) / N)
+        Fi = np.conj(F) / N
+        for sh in (np.exp(1j * k * h / 2), np.exp(-1j * k * h / 2)):
+            D = 1j * k * sh
+            if N % 2 == 0:
+                D[N // 2] = D[N // 2].real          # irfft keeps only Re of the Nyquist bin
+            M = (Fi @ np.diag(D) @ F).real
+            f = np.random.default_rng(N).standard_normal((N, 3))
+            assert np.allclose(O.spectral_gradient(f, k, sh, 0), M @ f, atol=1e-12)
+
+
+# ---------------------------------------------------------------- whole loop: exact recurrence
+def _golden_case(name):
+    g = np.load(os.path.join(GOLD, name + ".npz"))
+    shape = tuple(int(v) for v in g["shape"])
+    nd = len(shape)
+    args = []
+    for n in shape:
+        args += [n, float(g["d"])]
+    kg = K.KWaveGrid(*args)
+    kg.dt, kg.Nt = float(g["dt"]), int(g["Nt"])
+    mode = str(g["alpha_mode"])
+    med = K.KWaveMedium(sound_speed=float(g["c0"]), density=float(g["rho0"]),
+                        **({} if not mode else dict(alpha_coeff=float(g["alpha_coeff"]),
+                                                      alpha_power=float(g["alpha_power"]), alpha_mode=mode)))
+    pm = np.zeros(int(np.prod(shape)), bool)
+    pm[g["src_idx"]] = True
+    sm = np.zeros(int(np.prod(shape)), bool)
+    sm[g["sensor_idx"]] = True
+    kw = {}
+    if "p0" in g:
+        kw["p0"] = g["p0"].reshape(shape, order="F")
+    src = K.KWaveSource(p_mask=pm.reshape(shape, order="F"), p=g["src_signal"], **kw)
+    sen = K.KWaveSensor(mask=sm.reshape(shape, order="F"), record=["p", "p_final"])
+    return g, kg, med, src, sen
+
+
+GOLDEN = ["rec2d_lossless", "rec2d_p0", "rec2d_nodisp", "rec2d_stokes", "rec3d_lossless", "rec3d_stokes_p0",
+          "rec1d_p0"]
+
+
+@pytest.mark.parametrize("name", GOLDEN)
+def test_oracle_matches_exact_recurrence(name):
+    g, kg, med, src, sen = _golden_case(name)
+    out = O.kspace_first_order(kg, med, src, sen, pml_size=0, smooth_p0=False)
+    assert rel_l2(out["p"], g["p"]) < 1e-12
+    assert rel_l2(out["p_final"], g["p_final_field"][g["sensor_idx"]]) < 1e-12
+
+
+def test_planted_mistakes_are_caught(monkeypatch):
+    """Plant plausible misreadings in the oracle; the recurrence must reject each."""
+    g, kg, med, src, sen = _golden_case("rec2d_lossless")
+    base = O.spectral_gradient
+
+    def unshifted_divergence(f, k, shift, dim):   # grad p staggered, div u collocated
+        neg = shift.size > 1 and shift[1].imag < 0
+        return base(f, k, np.ones_like(shift) if neg else shift, dim)
+
+    def no_shift(f, k, shift, dim):               # drop the staggered shift entirely
+        return base(f, k, np.ones_like(shift), dim)
+
+    # (reversing both shifts leaves D- D+ = -k^2 unchanged; the analytic
+    #  staggered-derivative test above pins their direction)
+    for bad in (unshifted_divergence, no_shift):
+        monkeypatch.setattr(O, "spectral_gradient", bad)
+        out = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, pml_size=0, smooth_p0=False)
+        assert rel_l2(out["p"], g["p"]) > 1e-3, bad.__name__
+    monkeypatch.setattr(O, "spectral_gradient", base)
+    kap = O.compute_kappa
+    monkeypatch.setattr(O, "compute_kappa", lambda k, c, dt: np.ones_like(k))   # forget kappa
+    out = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, pml_size=0, smooth_p0=False)
+    assert rel_l2(out["p"], g["p"]) > 1e-4
+    monkeypatch.setattr(O, "compute_kappa", kap)
+    # source scaling p/(N c^2): drop the 1/N
+    sig2 = K.KWaveSource(p_mask=src.p_mask, p=src.p * 2.0)
+    out = O.kspace_first_order(copy.deepcopy(kg), med, sig2, sen, pml_size=0, smooth_p0=False)
+    assert rel_l2(out["p"], g["p"]) > 0.5
+
+
+# ---------------------------------------------------------------- branch identities
+def test_uniform_maps_equal_scalar_branches():
+    """Heterogeneous branches with uniform maps reproduce the scalar branches
+    (first_order_steps.jl:232-289); heterogeneous c0 drops dispersion (App. A-6)."""
+    kg, _, src, sen, kw = scenario("psrc", (32, 24), nt=40)
+    shape = (32, 24)
+    for extra, other in (({}, {}), (dict(alpha_coeff=5e3, alpha_power=1.5, alpha_mode="no_dispersion"),
+                                     dict(alpha_coeff=5e3, alpha_power=1.5, alpha_mode="stokes"))):
+        m_s = K.KWaveMedium(sound_speed=1500.0, density=1000.0, BonA=5.0, **extra)
+        m_a = K.KWaveMedium(sound_speed=np.full(shape, 1500.0), density=np.full(shape, 1000.0),
+                            BonA=5.0, **(other or extra))
+        a = O.kspace_first_order(copy.deepcopy(kg), m_s, src, sen, **kw)
+        b = O.kspace_first_order(copy.deepcopy(kg), m_a, src, sen, **kw)
+        assert rel_l2(b["p"], a["p"]) < 1e-12
+
+
+def test_one_step_dense_brute_force_heterogeneous_nonlinear():
+    """One step on a 6x5 grid from a random state, every operator a dense
+    matrix from explicit DFT sums (no FFT library, no rfft slicing)."""
+    shape = (6, 5)
+    rng = np.random.default_rng(3)
+    kg = K.KWaveGrid(6, 1e-4, 5, 1.2e-4)
+    c0 = 1500 + 100 * rng.random(shape)
+    rho0 = 1000 + 100 * rng.random(shape)
+    K.make_time(kg, c0)
+    kg.Nt = 1
+    med = K.KWaveMedium(sound_speed=c0, density=rho0, BonA=6.0, alpha_coeff=3e3, alpha_power=1.4,
+                        alpha_mode="no_dispersion")
+    p0 = rng.standard_normal(shape) * 1e5
+    src = K.KWaveSource(p0=p0)
+    sen = K.KWaveSensor(mask=np.ones(shape, bool), record=["p"])
+    out = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, pml_size=2, pml_alpha=3e6, smooth_p0=False)
+    # dense restatement
+    dt = kg.dt
+    ks = [O.make_wavenumber_vec(6, 1e-4), O.make_wavenumber_vec(5, 1.2e-4)]
+    hs = [1e-4, 1.2e-4]
+
+    def dft(N):
+        n = np.arange(N)
+        return np.exp(-2j * math.pi * np.outer(n, n) / N)
+
+    def deriv(a, sign):
+        N = shape[a]
+        D = 1j * ks[a] * np.exp(sign * 1j * ks[a] * hs[a] / 2)
+        if N % 2 == 0:
+            D[N // 2] = D[N // 2].real
+        F = dft(N)
+        M = (np.conj(F) / N @ np.diag(D) @ F).real
+        return lambda f: np.moveaxis(np.tensordot(M, np.moveaxis(f, a, 0), axes=1), 0, a)
+
+    pml = [O.get_pml(n, h, 2, 3e6, dt) for n, h in zip(shape, hs)]
+    pmls = [O.get_pml(n, h, 2, 3e6, dt, staggered=True) for n, h in zip(shape, hs)]
+    bs = lambda v, a: v.reshape([-1 if i == a else 1 for i in range(2)])
+    u = [-dt / (2 * O.stagger_density(rho0, a)) * deriv(a, +1)(p0) for a in range(2)]
+    rho = [p0 / (2 * c0 ** 2) for _ in range(2)]
+    for a in range(2):
+        u[a] = bs(pmls[a], a) * (bs(pmls[a], a) * u[a] - dt / O.stagger_density(rho0, a) * deriv(a, +1)(p0))
+    for a in range(2):
+        rho[a] = bs(pml[a], a) * (bs(pml[a], a) * rho[a] - dt * rho0 * deriv(a, -1)(u[a]))
+    r = rho[0] + rho[1]
+    f = c0 ** 2 * r * (1 + 6.0 / 2 * r / rho0)
+    kk = np.sqrt(ks[0][:, None] ** 2 + ks[1][None, :] ** 2)
+    c_ref = float(np.mean(c0))
+    x = c_ref * kk * dt / 2
+    kap = np.where(x == 0, 1.0, np.sin(x) / np.where(x == 0, 1, x))
+    tau = -2 * O.db2neper(3e3, 1.4) * c_ref ** 0.4
+    Mk = kap * (1 + tau * kk ** 1.4)
+    F0, F1 = dft(6), dft(5)
+    S = F0 @ f @ F1.T
+    p1 = (np.conj(F0) / 6 @ (Mk * S) @ np.conj(F1).T / 5).real
+    assert np.allclose(out["p"][:, 0], p1.ravel(order="F"), rtol=1e-11, atol=1e-9 * np.abs(p1).max())
+
+
+# ---------------------------------------------------------------- invariants from the reference's solver tests
+def test_solver_invariants_from_reference_tests():
+    kg = K.KWaveGrid(32, 1e-4, 32, 1e-4)
+    K.make_time(kg, 1500.0, t_end=1e-6)
+    med = K.KWaveMedium(sound_speed=1500.0, density=1000.0)
+    src = K.KWaveSource(p0=O.make_disc(32, 32, 16, 16, 3).astype(float))
+    m = np.zeros((32, 32), bool)
+    m[15, 15] = True
+    out = O.kspace_first_order(kg, med, src, K.KWaveSensor(mask=m, record=["p", "p_max"]), smooth_p0=False)
+    assert out["p_max"][0] >= out["p"][0].max() - 1e-10                       # test_solver_2d.jl:102-122
+    kg1 = K.KWaveGrid(64, 1e-4, 64, 1e-4)
+    K.make_time(kg1, 1500.0, t_end=5e-6)
+    p0 = O.make_disc(64, 64, 32, 32, 5).astype(float)
+    sen = K.KWaveSensor(mask=np.ones((64, 64), bool), record=["p_rms"])
+    lossless = O.kspace_first_order(copy.deepcopy(kg1), K.KWaveMedium(sound_speed=1500.0, density=1000.0),
+                                    K.KWaveSource(p0=p0), sen, smooth_p0=False)
+    absorbing = O.kspace_first_order(copy.deepcopy(kg1), K.KWaveMedium(
+        sound_speed=1500.0, density=1000.0, alpha_coeff=2.0, alpha_power=1.5, alpha_mode="no_dispersion"),
+        K.KWaveSource(p0=p0), sen, smooth_p0=False)
+    assert np.sum(absorbing["p_rms"] ** 2) < np.sum(lossless["p_rms"] ** 2)   # test_absorption.jl:46-82
+
+
+def test_time_reversal_peak_invariant():
+    """test_time_reversal.jl:2-54: TR of boundary data peaks within 5 cells of the source."""
+    kg = K.KWaveGrid(64, 1e-4, 64, 1e-4)
+    K.make_time(kg, 1500.0, t_end=5e-6)
+    med = K.KWaveMedium(sound_speed=1500.0, density=1000.0)
+    m = np.zeros((64, 64), bool)
+    m[0, :] = m[-1, :] = m[:, 0] = m[:, -1] = True
+    fwd = O.kspace_first_order(copy.deepcopy(kg), med, K.KWaveSource(p0=O.make_disc(64, 64, 32, 32, 3).astype(float)),
+                               K.KWaveSensor(mask=m, record=["p"]), smooth_p0=False)
+    tr = O.kspace_first_order(copy.deepcopy(kg), med, K.KWaveSource(),
+                              K.KWaveSensor(mask=m, time_reversal_boundary_data=fwd["p"], record=["p_final"]),
+                              smooth_p0=False)
+    pg = tr["p_final"].reshape(64, 64, order="F")
+    i, j = np.unravel_index(np.argmax(np.abs(pg)), pg.shape)
+    assert abs(i + 1 - 32) <= 5 and abs(j + 1 - 32) <= 5
+
+
+def test_smooth_window_and_restore_max():
+    """filters.jl:20-73: Blackman closed form; restore_max keeps max|.|; a
+    constant field is scaled by the DC weight circshift(w, N/2)[0] = w[N - N/2]."""
+    w = O.blackman_window(9)
+    n = np.arange(9)
+    assert np.allclose(w, 0.42 - 0.5 * np.cos(2 * math.pi * n / 8) + 0.08 * np.cos(4 * math.pi * n / 8))
+    x = np.random.default_rng(0).standard_normal((16, 12))
+    s = O.smooth(x, restore_max=True)
+    assert np.max(np.abs(s)) == pytest.approx(np.max(np.abs(x)))
+    c = np.full((16, 13), 2.5)
+    assert np.allclose(O.smooth(c), c * O.blackman_window(16)[8] * O.blackman_window(13)[7])
