@@ -1,0 +1,171 @@
+"""Benchmark of the k-space first-order PSTD time loop on one MI355X.
+
+Workload (one "step" = one PSTD time step of the whole grid): the 3-D
+heterogeneous absorbing nonlinear configuration -- 256^3, dx = 0.1 mm,
+PML 20, 8 seeded ellipsoids in c0/rho0, alpha0 = 0.75 dB/(MHz^y cm),
+y = 1.5, B/A = 5, plane pressure source, full x-y sensor plane recording
+p, p_max and p_rms, Float32 kernels.  Inputs are resident in HBM before the
+timed region; the timed region is K steps between two device syncs.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints ONE JSON line (metric, value = voxel-updates/s, roofline of the
+dominant kernel measured with HIP events on the simulation stream, and the
+oracle timed on the host cores as the CPU baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "k-wave-julia_amd"))
+
+METRIC = "voxel-updates/s + % HBM roofline, FP32 vs FP64 form at ≤1e-4 rel-L2"
+HBM_PEAK_GBS = 8000.0
+
+
+def kernel_bytes_per_voxel(name: str, nx: int, s: int = 4) -> float:
+    """Algorithmic (compulsory) bytes per grid voxel of one launch of each kernel.
+
+    Directional passes read p, u_d, rho_d, rho0 and write u_d, rho_d; the x
+    pass also reads rho_y, rho_z, c0^2 and writes the half spectrum
+    (Nx/2+1 complex per Nx reals); spectral passes read and write the half
+    spectrum; the c2r pass reads it and writes p.
+    """
+    half = 2 * s * (nx // 2 + 1) / nx
+    return {
+        "k_dir_pass_y": 6 * s,
+        "k_dir_pass_z": 6 * s,
+        "k_dir_pass_x_eos": 9 * s + half,
+        "k_spec_pass_y_fwd": 2 * half,
+        "k_spec_pass_z_mul": 2 * half,
+        "k_spec_pass_y_inv": 2 * half,
+        "k_c2r_x": half + s,
+    }.get(name, 0.0)
+
+
+def cpu_baseline(n_steps_max: int = 20, budget_s: float = 20.0):
+    """Oracle (FP32 restatement, scipy.fft threads) on the same workload, truncated."""
+    import numpy as np
+    from oracle import kwave_oracle as O
+    from kwave_amd.configs import config2
+
+    workers = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    workers = min(workers, 16)
+    O.set_fft_workers(workers)
+    kg, med, src, sen, kw = config2(Nt=1200)
+    stamps = []
+    t_start = time.perf_counter()
+
+    class Stop(Exception):
+        pass
+
+    def cb(t, Nt, p):
+        stamps.append(time.perf_counter())
+        # step 1 is the warm-up; stop once the budget or the step cap is used
+        if len(stamps) >= 2 and (stamps[-1] - stamps[0] > budget_s or len(stamps) > n_steps_max):
+            raise Stop()
+
+    try:
+        O.kspace_first_order(kg, med, src, sen, pml_size=kw["pml_size"], pml_alpha=kw["pml_alpha"],
+                             smooth_p0=False, data_cast=np.float32, progress_callback=cb)
+    except Stop:
+        pass
+    n = len(stamps) - 1
+    dt = stamps[-1] - stamps[0]
+    vox = 256 ** 3
+    return {"value": vox * n / dt, "unit": "voxel-updates/s", "cores": workers, "kind": "port",
+            "sample": f"oracle FP32 restatement of the 256^3 workload: 1 warm-up step + {n} timed steps "
+                      f"({dt:.1f} s; scipy.fft with {workers} threads, NumPy elementwise on 1 thread), "
+                      f"setup {stamps[0] - t_start:.1f} s excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--N", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU slab decomposition "
+              "is a design, not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import kwave_amd as K
+    from kwave_amd.configs import config2, config2_bytes_per_voxel_update
+
+    t0 = time.perf_counter()
+    nt = a.warmup + a.steps + a.profile_steps
+    kg, med, src, sen, kw = config2(N=a.N, Nt=nt)
+    desc, keep, info = K.build_desc(kg, med, src, sen, pml_size=kw["pml_size"], pml_alpha=kw["pml_alpha"],
+                                    smooth_p0=False, data_cast=np.float32)
+    sim = K.Simulation(desc, keep)
+    setup_s = time.perf_counter() - t0
+    print(f"[bench] setup {setup_s:.1f}s; warmup {a.warmup} steps", file=sys.stderr, flush=True)
+    sim.run(a.warmup)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    sim.run(a.steps)
+    sim.synchronize()
+    t2 = time.perf_counter()
+    print(f"[bench] timed {a.steps} steps in {t2 - t1:.3f}s", file=sys.stderr, flush=True)
+    prof = sim.profile(a.profile_steps)
+    sim.synchronize()
+    pmax = sim.sensor("p_max")
+    if not np.all(np.isfinite(pmax)):
+        raise RuntimeError("non-finite sensor data")
+    sim.close()
+
+    N = a.N
+    vox = N ** 3
+    secs = t2 - t1
+    value = vox * a.steps / secs
+    # dominant kernel = largest share of the step time
+    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
+    dname, dstat = dom
+    bpv = kernel_bytes_per_voxel(dname, N)
+    achieved = bpv * vox / (dstat["ms"] * 1e-3) / 1e9
+    step_b = config2_bytes_per_voxel_update(4)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": secs * 1e3 / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"3D heterogeneous absorbing nonlinear PSTD, {N}^3, dx=0.1mm, PML 20, "
+                               f"8 seeded ellipsoids, alpha0=0.75 y=1.5 B/A=5, plane p source, "
+                               f"z-plane sensor p/p_max/p_rms",
+                   "grid": [N, N, N], "precision": "fp32", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": bpv * vox, "avg_launch_ms": dstat["ms"]},
+        "step_roofline": {"bytes_per_voxel_update": step_b, "achieved": step_b * value / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_b * value / 1e9 / HBM_PEAK_GBS},
+        "kernels_ms": {k: round(v["ms"], 5) for k, v in prof.items()},
+        "setup_s": setup_s,
+    }
+    if not a.no_cpu:
+        print("[bench] cpu baseline (oracle) ...", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline()
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -126,6 +126,15 @@ typedef struct kw_sim_desc {
   /* time reversal (first_order.jl:426-433, 450-460): n_sensor x Nt data injected
    * (Dirichlet) at sensor_index, reversed in time; NULL if not TR */
   const double* tr_data;
+
+  /* sensor post-filters on the recorded p (first_order_steps.jl:836-910) */
+  int32_t directivity;              /* directivity_angle and directivity_size both set */
+  const double* directivity_angle;  /* per-sensor angles [rad] or NULL (then the scalar) */
+  double directivity_angle_scalar;
+  double directivity_size;          /* [m] */
+  int32_t freq_response;            /* frequency_response set */
+  double freq_center;               /* [Hz] */
+  double freq_bandwidth;            /* [Hz] */
 } kw_sim_desc;
 
 /* Library identity and errors. */

@@ -254,7 +254,12 @@ class Sim final : public SimBase {
     }
     switch (f) {
       case KW_REC_P:
-        copy_out(p_ts_.as<T>(), rows * cols, out);
+        if ((dir_ || fr_) && t_ == Nt_) {
+          apply_post_filters();
+          HIPC(hipMemcpy(out, pfilt_.p, rows * cols * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+          copy_out(p_ts_.as<T>(), rows * cols, out);
+        }
         break;
       case KW_REC_UX:
       case KW_REC_UY:
@@ -372,8 +377,8 @@ class Sim final : public SimBase {
       if (N_[a] < 1) raise(KW_ERR_INVALID, "grid size must be positive");
       if (!size_supported(N_[a]))
         raise(KW_ERR_UNSUPPORTED, "axis length " + std::to_string(N_[a]) +
-                                      " has no line-FFT plan (supported: 8,12,16,24,32,48,64,96,128,192,256,"
-                                      "384,512,768,1024,2048,4096)");
+                                      " has no line-FFT plan (supported: 8,10,12,16,20,24,32,40,48,64,80,96,"
+                                      "128,160,192,256,320,384,512,640,768,1024,2048,4096)");
       if (d.pml_size[a] < 0) raise(KW_ERR_INVALID, "pml_size must be >= 0");
       if (2LL * d.pml_size[a] > N_[a] + 0 && d.pml_size[a] > N_[a])
         raise(KW_ERR_INVALID, "pml_size larger than the grid");
@@ -612,6 +617,19 @@ class Sim final : public SimBase {
   void build_sensor(const kw_sim_desc& d) {
     nsens_ = d.n_sensor;
     rec_ = d.record;
+    dir_ = d.directivity != 0;
+    fr_ = d.freq_response != 0;
+    if (dir_) {
+      dir_size_ = d.directivity_size;
+      dir_ang_s_ = d.directivity_angle_scalar;
+      if (d.directivity_angle && d.n_sensor > 0)
+        upload(dir_ang_, std::vector<double>(d.directivity_angle, d.directivity_angle + d.n_sensor));
+    }
+    if (fr_) {
+      fc_ = d.freq_center;
+      bw_ = d.freq_bandwidth;
+    }
+    dmin_ = nd_ == 1 ? d.d[0] : *std::min_element(d.d, d.d + nd_);
     cartesian_ = d.n_sensor > 0 && d.sensor_index == nullptr;
     if (tr_) {
       // TR injection values p_TR[j, t] / (ndim c^2) (first_order.jl:234-242, 450-460, 678-689)
@@ -727,6 +745,7 @@ class Sim final : public SimBase {
       Y.M1 = M1_.as<T>();
       Y.M2 = stokes_ ? M2_.as<T>() : nullptr;
       Y.NH = (int)NH_;
+      Y.Sc = 1;
       Y.Sa = NH_;
       Y.So = NH_ * N_[1];
       Y.x_blocks = (int)((NH_ + spec_[0].lpw - 1) / spec_[0].lpw);
@@ -745,6 +764,7 @@ class Sim final : public SimBase {
       Z.M1 = M1_.as<T>();
       Z.M2 = stokes_ ? M2_.as<T>() : nullptr;
       Z.NH = (int)NH_;
+      Z.Sc = 1;
       Z.Sa = NH_ * N_[1];
       Z.So = NH_;
       Z.x_blocks = (int)((NH_ + spec_[1].lpw - 1) / spec_[1].lpw);
@@ -787,9 +807,9 @@ class Sim final : public SimBase {
 
   // initialize_p0_{1,2,3}d! (first_order_steps.jl:58-191) with p0 already smoothed
   void init_p0(const kw_sim_desc& d) {
-    if (d.smooth_p0) raise(KW_ERR_UNSUPPORTED, "smooth_p0 is applied by the host wrapper before kw_sim_create");
     DevBuf p0;
     upload(p0, to_t<T>(std::vector<double>(d.p0, d.p0 + nvox_)));
+    if (d.smooth_p0) smooth_p0(p0);
     // rho_d = p0 / (ndim c0^2); p = p0
     const unsigned nb = (unsigned)((nvox_ + 255) / 256);
     HIPC(launch_init_rho<T>(nb, stream_, p0.as<T>(), c0sq_.p ? c0sq_.as<T>() : nullptr,
@@ -803,6 +823,75 @@ class Sim final : public SimBase {
       HIPC(launch_dir<T>(cfg_[a].n, a == 0, cfg_[a].grid, cfg_[a].block, cfg_[a].shmem, stream_, A));
     }
     HIPC(hipStreamSynchronize(stream_));
+  }
+
+  // smooth(p0; restore_max=true) (filters.jl:20-60): per axis FFT, multiply
+  // by fftshift(blackman(N)), IFFT on complex data; then rescale so that
+  // max|smoothed| = max|p0|.
+  void smooth_p0(DevBuf& p0) {
+    DevBuf z, mx;
+    z.alloc(nvox_ * sizeof(Cx<T>));
+    mx.alloc(2 * sizeof(unsigned long long));
+    HIPC(hipMemsetAsync(mx.p, 0, mx.bytes, stream_));
+    const unsigned nb = (unsigned)((nvox_ + 255) / 256);
+    HIPC(launch_smooth_helpers<T>(0, nb, stream_, p0.as<T>(), z.as<Cx<T>>(), nullptr, nullptr, nvox_));
+    for (int a = 0; a < nd_; ++a) {
+      const long long n = N_[a];
+      std::vector<double> w(n), ws(n);
+      for (long long i = 0; i < n; ++i)
+        w[i] = n == 1 ? 1.0 : 0.42 - 0.5 * std::cos(2 * M_PI * i / (double)(n - 1)) +
+                                  0.08 * std::cos(4 * M_PI * i / (double)(n - 1));
+      for (long long i = 0; i < n; ++i) ws[i] = w[((i - n / 2) % n + n) % n] / (double)n;  // circshift(w, n/2)
+      DevBuf wb;
+      upload(wb, to_t<T>(ws));
+      SpecArgs<T> A;
+      std::memset(&A, 0, sizeof(A));
+      A.S1 = z.as<Cx<T>>();
+      A.tw = tw_[a].as<Cx<T>>();
+      A.mode = 3;
+      A.W = wb.as<T>();
+      long long lines, outer;
+      if (a == 0) {
+        lines = N_[1] * N_[2];
+        A.Sc = N_[0];
+        A.Sa = 1;
+        A.So = 0;
+        outer = 1;
+      } else {
+        lines = N_[0];
+        A.Sc = 1;
+        A.Sa = a == 1 ? N_[0] : N_[0] * N_[1];
+        A.So = a == 1 ? N_[0] * N_[1] : N_[0];
+        outer = a == 1 ? N_[2] : N_[1];
+      }
+      A.NH = (int)lines;
+      PassCfg c = line_cfg(n, lines, outer);
+      A.x_blocks = (int)((lines + c.lpw - 1) / c.lpw);
+      HIPC(launch_spec<T>((int)n, c.grid, c.block, c.shmem, stream_, A));
+      HIPC(hipStreamSynchronize(stream_));
+    }
+    const unsigned nr = std::min<unsigned>(nb, 1024u);
+    HIPC(launch_smooth_helpers<T>(1, nr, stream_, p0.as<T>(), z.as<Cx<T>>(), mx.as<unsigned long long>(), nullptr,
+                                  nvox_));
+    HIPC(launch_smooth_helpers<T>(2, nb, stream_, nullptr, z.as<Cx<T>>(), mx.as<unsigned long long>(), p0.as<T>(),
+                                  nvox_));
+    HIPC(hipStreamSynchronize(stream_));
+  }
+
+  // launch geometry for complex lines of length n (lines x outer)
+  PassCfg line_cfg(long long n, long long lines, long long outer) {
+    PassCfg c;
+    c.n = (int)n;
+    const int tpl = plan_tpl((int)n);
+    const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
+    int lpw = 256 / tpl;
+    while (lpw > 1 && (size_t)lpw * line_bytes > 65536) lpw /= 2;
+    while (lpw > 1 && lpw / 2 >= lines) lpw /= 2;
+    c.lpw = lpw;
+    c.block = dim3(lpw * tpl);
+    c.shmem = (size_t)lpw * line_bytes;
+    c.grid = dim3((unsigned)(((lines + lpw - 1) / lpw) * std::max<long long>(outer, 1)));
+    return c;
   }
 
   // ---- one time step ------------------------------------------------------
@@ -849,6 +938,52 @@ class Sim final : public SimBase {
     }
   }
 
+  // apply_sensor_directivity! then apply_frequency_response! on the recorded
+  // p (first_order_steps.jl:724-725, 836-910); computed once, in FP64.
+  void apply_post_filters() {
+    if (filtered_) return;
+    const long long ns = nsens_, nt = Nt_alloc_;
+    std::vector<double> cw(nt), sw(nt), absf(nt);
+    for (long long m = 0; m < nt; ++m) {
+      long long mm = m;
+      double sg = -1.0;
+      if (2 * mm > nt) {
+        mm = nt - mm;
+        sg = 1.0;
+      }
+      cw[m] = std::cos(2.0 * M_PI * (double)mm / (double)nt);
+      sw[m] = sg * std::sin(2.0 * M_PI * (double)mm / (double)nt);
+      // filters.jl:158-165: [0..N/2-1, -N/2..-1] (odd: symmetric) * Fs/N
+      const long long k = (nt % 2 == 0) ? (m < nt / 2 ? m : m - nt) : (m <= (nt - 1) / 2 ? m : m - nt);
+      absf[m] = std::fabs((double)k * ((1.0 / dt_) / (double)nt));
+    }
+    DevBuf dcw, dsw, dabs, Y;
+    upload(dcw, cw);
+    upload(dsw, sw);
+    upload(dabs, absf);
+    Y.alloc((size_t)ns * nt * sizeof(Cx<double>));
+    pfilt_.alloc((size_t)ns * nt * sizeof(double));
+    FiltArgs f;
+    f.ns = (int)ns;
+    f.nt = (int)nt;
+    f.cw = dcw.as<double>();
+    f.sw = dsw.as<double>();
+    f.absf = dabs.as<double>();
+    f.ang = dir_ang_.p ? dir_ang_.as<double>() : nullptr;
+    f.ang_s = dir_ang_s_;
+    f.dir = dir_ ? 1 : 0;
+    f.dsize = dir_size_;
+    f.kscale = 2.0 * M_PI / dmin_;
+    f.fr = fr_ ? 1 : 0;
+    f.fc = fc_;
+    f.bw = bw_;
+    f.Y = Y.as<Cx<double>>();
+    f.out = pfilt_.as<double>();
+    HIPC(launch_filter<T>(stream_, p_ts_.as<T>(), f));
+    HIPC(hipStreamSynchronize(stream_));
+    filtered_ = true;
+  }
+
   const long long* sidx_tr() {
     if (!sidx_.p) {
       // TR with nothing recorded still needs the injection indices
@@ -890,6 +1025,10 @@ class Sim final : public SimBase {
   uint32_t rec_ = 0;
   DevBuf sidx_, p_ts_, u_ts_[3], p_max_, p_min_, p_rms_, p_final_, u_max_, u_rms_, u_final_, I_avg_, I_max_;
   DevBuf tr_vals_;
+  // post-filters
+  bool dir_ = false, fr_ = false, filtered_ = false;
+  double dir_size_ = 0, dir_ang_s_ = 0, fc_ = 0, bw_ = 1, dmin_ = 1;
+  DevBuf dir_ang_, pfilt_;
 };
 
 template <typename T>

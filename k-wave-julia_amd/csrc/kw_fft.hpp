@@ -240,18 +240,25 @@ struct Plan;
     using R = Radices<__VA_ARGS__>;                \
   };
 KW_PLAN(8, 1, 8)
+KW_PLAN(10, 1, 10)
 KW_PLAN(12, 1, 12)
 KW_PLAN(16, 1, 16)
+KW_PLAN(20, 2, 10, 2)
 KW_PLAN(24, 2, 12, 2)
 KW_PLAN(32, 4, 8, 4)
+KW_PLAN(40, 2, 20, 2)
 KW_PLAN(48, 4, 12, 4)
 KW_PLAN(64, 8, 8, 8)
+KW_PLAN(80, 4, 20, 4)
 KW_PLAN(96, 4, 24, 4)
 KW_PLAN(128, 8, 16, 8)
+KW_PLAN(160, 8, 20, 4, 2)
 KW_PLAN(192, 8, 24, 8)
 KW_PLAN(256, 16, 16, 16)
+KW_PLAN(320, 16, 20, 4, 4)
 KW_PLAN(384, 16, 24, 8, 2)
 KW_PLAN(512, 16, 32, 16)
+KW_PLAN(640, 32, 20, 4, 4, 2)
 KW_PLAN(768, 32, 24, 8, 4)
 KW_PLAN(1024, 32, 32, 32)
 KW_PLAN(2048, 64, 32, 8, 8)

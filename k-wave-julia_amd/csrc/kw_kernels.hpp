@@ -97,11 +97,12 @@ struct LineCtx {
   __device__ __forceinline__ long long ib(int q) const { return bb + sa * (long long)(t + TPL * q); }
   __device__ __forceinline__ int n(int q) const { return t + TPL * q; }
 
-  __device__ __forceinline__ void load(const T* __restrict__ f, Cx<T> (&v)[E]) const {
+  // `fill` is what an absent line of the pair holds (1 for divisors like rho0)
+  __device__ __forceinline__ void load(const T* __restrict__ f, Cx<T> (&v)[E], T fill = (T)0) const {
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      v[q].x = va ? f[ia(q)] : (T)0;
-      v[q].y = vb ? f[ib(q)] : (T)0;
+      v[q].x = va ? f[ia(q)] : fill;
+      v[q].y = vb ? f[ib(q)] : fill;
     }
   }
   __device__ __forceinline__ void store(T* __restrict__ f, const Cx<T> (&v)[E]) const {
@@ -112,13 +113,13 @@ struct LineCtx {
     }
   }
   // value at element n+1 (periodic) along the line
-  __device__ __forceinline__ void load_next(const T* __restrict__ f, Cx<T> (&v)[E]) const {
+  __device__ __forceinline__ void load_next(const T* __restrict__ f, Cx<T> (&v)[E], T fill = (T)0) const {
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       int nn = t + TPL * q + 1;
       nn = (nn == N) ? 0 : nn;
-      v[q].x = va ? f[ba + sa * (long long)nn] : (T)0;
-      v[q].y = vb ? f[bb + sa * (long long)nn] : (T)0;
+      v[q].x = va ? f[ba + sa * (long long)nn] : fill;
+      v[q].y = vb ? f[bb + sa * (long long)nn] : fill;
     }
   }
 };
@@ -210,8 +211,8 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
   Cx<T> r0[E];
   if (a.rho0 != nullptr) {
     Cx<T> r1[E];
-    L.load(a.rho0, r0);
-    L.load_next(a.rho0, r1);
+    L.load(a.rho0, r0, (T)1);
+    L.load_next(a.rho0, r1, (T)1);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       r1[q].x = (r0[q].x + r1[q].x) * (T)0.5;
@@ -400,11 +401,13 @@ struct SpecArgs {
   const Cx<T>* tw;
   const T* M1;        // octant multiplier tables or null
   const T* M2;
-  int NH;             // Nx/2+1 (lines along kx)
+  int NH;             // lines along the first index (Nx/2+1 for the EOS passes)
+  long long Sc;       // stride between those lines (1 for the EOS passes)
   long long Sa;       // stride along the line (complex elements)
   long long So;       // outer stride
   int x_blocks;
-  int mode;           // 0 fwd, 1 inv, 2 fwd*M*inv
+  int mode;           // 0 fwd, 1 inv, 2 fwd*M*inv, 3 fwd*W*inv (1-D window W along the line)
+  const T* W;         // mode 3 window (length N, 1/N folded in)
   // octant table addressing: index = kx + MH1*(|i1| + MH2*|i2|)
   int axis;           // 1 = line along y, 2 = line along z
   int Ny, Nz;
@@ -425,12 +428,17 @@ __global__ void __launch_bounds__(256) k_spec_pass(SpecArgs<T> a) {
   const int kx = xb * lpw + c;
   const bool valid = kx < a.NH;
   Cx<T>* lds = lds_all + c * P;
-  const long long base = (long long)kx + a.So * o;
+  const long long base = (long long)kx * a.Sc + a.So * o;
 
   Cx<T> v[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = valid ? a.S1[base + a.Sa * (t + TPL * q)] : Cx<T>{0, 0};
   if (a.mode == 1) {
+    line_fft<T, N, true>(v, lds, a.tw, t);
+  } else if (a.mode == 3) {
+    line_fft<T, N, false>(v, lds, a.tw, t);
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cscale(v[q], a.W[t + TPL * q]);
     line_fft<T, N, true>(v, lds, a.tw, t);
   } else {
     line_fft<T, N, false>(v, lds, a.tw, t);
@@ -672,6 +680,120 @@ template <typename D>
 __global__ void __launch_bounds__(256) k_finalize_rms(D* a, long long n, D nt) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = sqrt(a[i] / nt);
+}
+
+// ---------------------------------------------------------------------------
+// smooth(p0; restore_max=true) support (filters.jl:20-60): the per-axis
+// filtering runs through k_spec_pass mode 3 on a complex copy of p0.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) k_real_to_cx(const T* p, Cx<T>* z, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = Cx<T>{p[i], (T)0};
+}
+
+// mx[0] = max|p0|, mx[1] = max|Re z| as IEEE-754 double bit patterns
+// (non-negative doubles order like their unsigned bit patterns).
+template <typename T>
+__global__ void __launch_bounds__(256) k_maxabs2(const T* p, const Cx<T>* z, unsigned long long* mx, long long n) {
+  __shared__ double s0[256], s1[256];
+  double a = 0.0, b = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double u = fabs((double)p[i]), v = fabs((double)z[i].x);
+    a = u > a ? u : a;
+    b = v > b ? v : b;
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      s0[threadIdx.x] = s0[threadIdx.x] > s0[threadIdx.x + s] ? s0[threadIdx.x] : s0[threadIdx.x + s];
+      s1[threadIdx.x] = s1[threadIdx.x] > s1[threadIdx.x + s] ? s1[threadIdx.x] : s1[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicMax(&mx[0], (unsigned long long)__double_as_longlong(s0[0]));
+    atomicMax(&mx[1], (unsigned long long)__double_as_longlong(s1[0]));
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_smooth_finish(const Cx<T>* z, const unsigned long long* mx, T* p, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T om = (T)__longlong_as_double((long long)mx[0]);
+  const T sm = (T)__longlong_as_double((long long)mx[1]);
+  const T s = sm > (T)0 ? om / sm : (T)1;
+  p[i] = sm > (T)0 ? z[i].x * s : z[i].x;
+}
+
+// ---------------------------------------------------------------------------
+// Sensor post-filters (apply_sensor_directivity!, apply_frequency_response!;
+// first_order_steps.jl:836-910): y = Re(ifft(fft(x) .* H)) per sensor row,
+// H(f) = sinc(2 pi |f| / h_min * size * sin(angle) / 2) * exp(-(|f| - fc)^2 / (2 bw^2)).
+// Direct DFT in FP64 (any record length); one workgroup per sensor row.
+// cw/sw hold cos(2 pi m/Nt) and -sin(2 pi m/Nt).
+// ---------------------------------------------------------------------------
+struct FiltArgs {
+  int ns, nt;
+  const double* cw;
+  const double* sw;
+  const double* absf;    // |freqs| (filters.jl:158-165)
+  const double* ang;     // per-sensor angle or null
+  double ang_s;
+  int dir;
+  double dsize, kscale;  // directivity size, 2 pi / min spacing
+  int fr;
+  double fc, bw;
+  Cx<double>* Y;         // ns x nt spectra (row-major per sensor)
+  double* out;           // column-major ns x nt
+};
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_filter_fwd(const T* p_ts, FiltArgs a) {
+  const int j = blockIdx.x;
+  const double ang = a.dir ? (a.ang ? a.ang[j] : a.ang_s) : 0.0;
+  const bool use_dir = a.dir && ang != 0.0;  // angle == 0 skips this sensor (first_order_steps.jl:856-858)
+  for (int f = threadIdx.x; f < a.nt; f += blockDim.x) {
+    double re = 0.0, im = 0.0;
+    int m = 0;
+    for (int n = 0; n < a.nt; ++n) {
+      const double x = (double)p_ts[(long long)n * a.ns + j];
+      re += x * a.cw[m];
+      im += x * a.sw[m];
+      m += f;
+      if (m >= a.nt) m -= a.nt;
+    }
+    double H = 1.0;
+    if (use_dir) {
+      const double arg = a.kscale * a.absf[f] * a.dsize * sin(ang) / 2;
+      H *= (arg == 0.0) ? 1.0 : sin(arg) / arg;
+    }
+    if (a.fr) {
+      const double d = a.absf[f] - a.fc;
+      H *= exp(-(d * d) / (2 * a.bw * a.bw));
+    }
+    a.Y[(long long)j * a.nt + f] = Cx<double>{re * H, im * H};
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_filter_inv(FiltArgs a) {
+  const int j = blockIdx.x;
+  const Cx<double>* Y = a.Y + (long long)j * a.nt;
+  for (int n = threadIdx.x; n < a.nt; n += blockDim.x) {
+    double acc = 0.0;
+    int m = 0;
+    for (int f = 0; f < a.nt; ++f) {
+      const Cx<double> y = Y[f];
+      acc += y.x * a.cw[m] + y.y * a.sw[m];
+      m += n;
+      if (m >= a.nt) m -= a.nt;
+    }
+    a.out[(long long)n * a.ns + j] = acc / a.nt;
+  }
 }
 
 }  // namespace kw

@@ -7,7 +7,8 @@
 
 // Supported line-FFT lengths (kw_fft.hpp Plan<N>).
 #define KW_SIZES(X) \
-  X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(2048) X(4096)
+  X(8) X(10) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) \
+  X(384) X(512) X(640) X(768) X(1024) X(2048) X(4096)
 
 namespace kw {
 
@@ -30,5 +31,11 @@ template <typename T>
 hipError_t launch_init_rho(unsigned nb, hipStream_t s, const T* p0, const T* c0sq, T c0sq_s, T ndim, T* p, T* r0, T* r1,
                            T* r2, long long n);
 hipError_t launch_finalize_rms(unsigned nb, hipStream_t s, double* a, long long n, double nt);
+template <typename T>
+hipError_t launch_smooth_helpers(int which, unsigned nb, hipStream_t s, const T* p, Cx<T>* z, unsigned long long* mx,
+                                 T* out, long long n);
+
+template <typename T>
+hipError_t launch_filter(hipStream_t s, const T* p_ts, const FiltArgs& a);
 
 }  // namespace kw

@@ -57,10 +57,35 @@ hipError_t launch_finalize_rms(unsigned nb, hipStream_t s, double* a, long long 
   return hipGetLastError();
 }
 
+// which: 0 = p -> complex z, 1 = max-abs reduction, 2 = rescale into out
+template <typename T>
+hipError_t launch_smooth_helpers(int which, unsigned nb, hipStream_t s, const T* p, Cx<T>* z, unsigned long long* mx,
+                                 T* out, long long n) {
+  if (which == 0)
+    hipLaunchKernelGGL((k_real_to_cx<T>), dim3(nb), dim3(256), 0, s, p, z, n);
+  else if (which == 1)
+    hipLaunchKernelGGL((k_maxabs2<T>), dim3(nb), dim3(256), 0, s, p, z, mx, n);
+  else
+    hipLaunchKernelGGL((k_smooth_finish<T>), dim3(nb), dim3(256), 0, s, z, mx, out, n);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_filter(hipStream_t s, const T* p_ts, const FiltArgs& a) {
+  hipLaunchKernelGGL((k_filter_fwd<T>), dim3(a.ns), dim3(256), 0, s, p_ts, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_filter_inv<T>), dim3(a.ns), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+
 #define KW_INST(T)                                                                                                    \
   template hipError_t launch_record<T>(unsigned, hipStream_t, const RecArgs<T>&);                                  \
   template hipError_t launch_tr_inject<T>(unsigned, hipStream_t, T*, T*, T*, const long long*, const T*, int);      \
-  template hipError_t launch_init_rho<T>(unsigned, hipStream_t, const T*, const T*, T, T, T*, T*, T*, T*, long long);
+  template hipError_t launch_init_rho<T>(unsigned, hipStream_t, const T*, const T*, T, T, T*, T*, T*, T*, long long); \
+  template hipError_t launch_smooth_helpers<T>(int, unsigned, hipStream_t, const T*, Cx<T>*, unsigned long long*, T*, \
+                                               long long);                                                          \
+  template hipError_t launch_filter<T>(hipStream_t, const T*, const FiltArgs&);
 KW_INST(float)
 KW_INST(double)
 #undef KW_INST

@@ -83,6 +83,13 @@ class SimDesc(C.Structure):
         ("sensor_index", C.POINTER(C.c_int64)),
         ("record", C.c_uint32),
         ("tr_data", C.POINTER(C.c_double)),
+        ("directivity", C.c_int32),
+        ("directivity_angle", C.POINTER(C.c_double)),
+        ("directivity_angle_scalar", C.c_double),
+        ("directivity_size", C.c_double),
+        ("freq_response", C.c_int32),
+        ("freq_center", C.c_double),
+        ("freq_bandwidth", C.c_double),
     ]
 
 

@@ -1,0 +1,103 @@
+"""Synthetic inputs of the benchmark configurations (seeded, reproducible).
+
+A splitmix64 stream (seed 42) drives every random draw so the inputs are the
+same from any host language.  Only host-side array construction lives here.
+"""
+from __future__ import annotations
+
+import math
+from typing import Tuple
+
+import numpy as np
+
+from .builders import make_circle, make_disc, tone_burst
+from .model import KWaveGrid, KWaveMedium, KWaveSensor, KWaveSource, make_time
+
+
+class SplitMix64:
+    """splitmix64 (Steele, Lea, Flood 2014); uniform() uses the top 53 bits."""
+
+    def __init__(self, seed: int = 42):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self) -> int:
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self, lo: float = 0.0, hi: float = 1.0) -> float:
+        return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / 9007199254740992.0))
+
+    def integer(self, lo: int, hi: int) -> int:
+        """Uniform integer in [lo, hi]."""
+        return lo + int(self.uniform() * (hi - lo + 1))
+
+
+def config1(data_cast=np.float64):
+    """2-D 128x128, dx = 0.1 mm, homogeneous lossless; disc p source, circle sensor."""
+    kg = KWaveGrid(128, 1e-4, 128, 1e-4)
+    make_time(kg, 1500.0)
+    med = KWaveMedium(sound_speed=1500.0, density=1000.0)
+    dt, Nt = kg.dt, kg.Nt
+    sig = tone_burst(1 / dt, 1e6, 5)
+    sig = np.concatenate([sig, np.zeros(max(0, Nt - sig.size))])[:Nt]
+    pm = make_disc(128, 128, 64, 64, 5)
+    src = KWaveSource(p_mask=pm, p=np.repeat(sig[None, :], int(pm.sum()), axis=0))
+    sen = KWaveSensor(mask=make_circle(128, 128, 64, 64, 40), record=["p", "p_final"])
+    return kg, med, src, sen, dict(pml_size=20, pml_alpha=2.0, data_cast=data_cast)
+
+
+def config2(N: int = 256, Nt: int = 1200, pml: int = 20, amplitude: float = 1e6, seed: int = 42):
+    """3-D heterogeneous absorbing nonlinear FP32 workload (N^3, dx = 0.1 mm).
+
+    c0 = 1500 m/s plus 8 axis-aligned ellipsoids (centres in [pml+1, N-pml]^3,
+    integer semi-axes in [10, 40], c0 ~ U[1400,1650], rho0 ~ U[900,1100];
+    later ones overwrite earlier ones); alpha0 = 0.75, y = 1.5, :stokes
+    (heterogeneous c0 -> absorption without dispersion), B/A = 5; plane p
+    source at z = pml+1 driven by a 1 MHz 3-cycle Gaussian tone burst of
+    `amplitude` Pa; sensor plane z = N - 56 recording p, p_max, p_rms.
+    The signal matrix holds the burst plus one zero column: past the last
+    column the reference re-adds the last value (first_order_steps.jl:625),
+    so this equals zero padding to Nt.
+    """
+    rng = SplitMix64(seed)
+    kg = KWaveGrid(N, 1e-4, N, 1e-4, N, 1e-4)
+    c0 = np.full((N, N, N), 1500.0)
+    rho0 = np.full((N, N, N), 1000.0)
+    lo, hi = pml + 1, N - pml
+    x = np.arange(1, N + 1)
+    for _ in range(8):
+        cen = [rng.uniform(lo, hi) for _ in range(3)]
+        ax = [rng.integer(10, 40) for _ in range(3)]
+        cv = rng.uniform(1400.0, 1650.0)
+        rv = rng.uniform(900.0, 1100.0)
+        sl = [slice(max(0, int(math.floor(c - a)) - 1), min(N, int(math.ceil(c + a)) + 1)) for c, a in zip(cen, ax)]
+        X = ((x[sl[0]] - cen[0]) / ax[0])[:, None, None] ** 2
+        Y = ((x[sl[1]] - cen[1]) / ax[1])[None, :, None] ** 2
+        Z = ((x[sl[2]] - cen[2]) / ax[2])[None, None, :] ** 2
+        inside = (X + Y + Z) <= 1.0
+        c0[sl[0], sl[1], sl[2]][inside] = cv
+        rho0[sl[0], sl[1], sl[2]][inside] = rv
+    make_time(kg, c0)
+    kg.Nt = Nt
+    kg.t_array = np.arange(Nt) * kg.dt
+    med = KWaveMedium(sound_speed=c0, density=rho0, alpha_coeff=0.75, alpha_power=1.5, alpha_mode="stokes",
+                      BonA=5.0)
+    burst = amplitude * tone_burst(1 / kg.dt, 1e6, 3)
+    pm = np.zeros((N, N, N), dtype=bool)
+    pm[:, :, pml] = True                                   # z = pml + 1 (1-based)
+    n_src = int(pm.sum())
+    sig = np.zeros((n_src, burst.size + 1))
+    sig[:, : burst.size] = burst[None, :]
+    src = KWaveSource(p_mask=pm, p=sig)
+    sm = np.zeros((N, N, N), dtype=bool)
+    sm[:, :, N - 57] = True                                # z = 200 for N = 256 (1-based)
+    sen = KWaveSensor(mask=sm, record=["p", "p_max", "p_rms"])
+    return kg, med, src, sen, dict(pml_size=pml, pml_alpha=2.0, data_cast=np.float32, smooth_p0=False)
+
+
+def config2_bytes_per_voxel_update(precision: int = 4) -> int:
+    """Compulsory bytes per voxel-update: 7 evolving fields read+written, 2 maps read."""
+    return 2 * 7 * precision + 2 * precision

@@ -271,6 +271,21 @@ def build_desc(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSourc
         trc = np.asfortranarray(trd)
         keep.append(trc)
         d.tr_data = trc.ctypes.data_as(_DP)
+    if sensor.directivity_angle is not None and sensor.directivity_size is not None:
+        d.directivity = 1
+        ang = np.asarray(sensor.directivity_angle, dtype=np.float64)
+        if ang.ndim == 0:
+            d.directivity_angle_scalar = float(ang)
+        else:
+            ang = np.ascontiguousarray(ang.ravel(order="F"))
+            if ang.size < info["n_sensor"]:
+                raise IndexError("directivity_angle has fewer entries than sensor points")
+            keep.append(ang)
+            d.directivity_angle = _dptr(ang)
+        d.directivity_size = float(sensor.directivity_size)
+    if sensor.frequency_response is not None:
+        d.freq_response = 1
+        d.freq_center, d.freq_bandwidth = sensor.frequency_response
     return d, keep, info
 
 
@@ -288,10 +303,6 @@ def kspace_first_order(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KW
     """
     if save_to_disk is not None:
         raise NotImplementedError("save_to_disk (HDF5 input writer) is not part of this build")
-    if sensor.directivity_angle is not None and sensor.directivity_size is not None:
-        raise NotImplementedError("sensor directivity post-filter is not part of this build yet")
-    if sensor.frequency_response is not None:
-        raise NotImplementedError("sensor frequency-response post-filter is not part of this build yet")
     desc, keep, info = build_desc(kgrid, medium, source, sensor, pml_size=pml_size, pml_alpha=pml_alpha,
                                   smooth_p0=smooth_p0, data_cast=data_cast)
     Nt = info["Nt"]

@@ -69,7 +69,7 @@ def test_parity_vs_oracle(name, shape, dtype):
     _compare(out, ref, TOL[dtype], f"{name}/{dtype.__name__}")
 
 
-SIZES = [8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512, 768, 1024, 2048, 4096]
+SIZES = [8, 10, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024, 2048, 4096]
 
 
 @pytest.mark.parametrize("n", SIZES)
@@ -105,3 +105,30 @@ def test_time_reversal_2d():
         print(f"    tr {dt.__name__} p_final rel-L2 {e:.3e}", flush=True)
         assert e <= tol
         assert out["p_final"].shape == (48 * 48,)
+
+
+@pytest.mark.parametrize("shape", [(64,), (48, 40), (24, 16, 20)])
+def test_smooth_p0(shape):
+    """initialize_p0 with smooth(p0; restore_max=true) (first_order_steps.jl:70-72, filters.jl:20-60)."""
+    print(f"\n[case] smooth_p0 {shape}", flush=True)
+    kg, med, src, sen, kw = scenario("hetc_p0_psrc", shape, nt=30)
+    kw["smooth_p0"] = True
+    ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
+    for dt, tol in ((np.float64, 1e-10), (np.float32, 1e-4)):
+        out = K.kspace_first_order(copy.deepcopy(kg), med, src, sen, data_cast=dt, **kw)
+        _compare(out, ref, tol, f"smooth-{shape}-{dt.__name__}")
+
+
+@pytest.mark.parametrize("angles", ["scalar", "per_sensor"])
+def test_sensor_post_filters(angles):
+    """apply_sensor_directivity! + apply_frequency_response! (first_order_steps.jl:836-910)."""
+    print(f"\n[case] post filters {angles}", flush=True)
+    kg, med, src, sen, kw = scenario("psrc", (40, 48), nt=45)
+    n = int(sen.mask.sum())
+    ang = 0.7 if angles == "scalar" else np.linspace(0.0, 1.2, n)   # angle 0 skips that sensor
+    sen2 = K.KWaveSensor(mask=sen.mask, record=["p", "p_max"], directivity_angle=ang,
+                         directivity_size=3e-4, frequency_response=(2.0e6, 1.5e6))
+    ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen2, **kw)
+    for dt, tol in ((np.float64, 1e-10), (np.float32, 1e-4)):
+        out = K.kspace_first_order(copy.deepcopy(kg), med, src, sen2, data_cast=dt, **kw)
+        _compare(out, ref, tol, f"filters-{angles}-{dt.__name__}")
