@@ -84,10 +84,13 @@ typedef struct kw_sim_desc {
   double dt;             /* kgrid.dt[] */
   int64_t Nt;            /* kgrid.Nt[] (time reversal: columns of the TR data) */
   int32_t precision;     /* 4 = Float32 kernels, 8 = Float64 kernels (data_cast) */
-  int32_t map_storage;   /* kw_map_storage */
+  int32_t map_storage;   /* kw_map_storage; KW_MAPS_FP16 stores c0 - c0_offset and
+                            rho0 - rho0_offset as FP16 (arithmetic stays in `precision`) */
   int32_t pml_size[3];   /* per axis (first_order.jl:322, 540) */
   double pml_alpha[3];
   double c_ref;          /* mean(c0) as the host computed it (first_order.jl:361, 581) */
+  double c0_offset;      /* KW_MAPS_FP16 only */
+  double rho0_offset;    /* KW_MAPS_FP16 only */
 
   const double* c0;      /* sound speed map or NULL (then c0_scalar) */
   double c0_scalar;

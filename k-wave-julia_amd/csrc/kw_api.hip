@@ -385,7 +385,9 @@ class Sim final : public SimBase {
     }
     if (d.dt == 0.0 || d.Nt == 0)
       raise(KW_ERR_INVALID, "Time array not set. Call make_time!(kgrid, sound_speed) before running.");
-    if (d.map_storage != KW_MAPS_NATIVE) raise(KW_ERR_UNSUPPORTED, "map storage mode not supported");
+    if (d.map_storage != KW_MAPS_NATIVE && d.map_storage != KW_MAPS_FP16)
+      raise(KW_ERR_INVALID, "unknown map storage mode");
+    maps_f16_ = d.map_storage == KW_MAPS_FP16;
     nvox_ = N_[0] * N_[1] * N_[2];
     dt_ = d.dt;
     Nt_ = d.Nt;
@@ -403,12 +405,17 @@ class Sim final : public SimBase {
       HIPC(hipMemset(u_[a].p, 0, u_[a].bytes));
       HIPC(hipMemset(rho_[a].p, 0, rho_[a].bytes));
     }
-    // medium maps
-    if (het_r) upload(rho0_, to_t<T>(std::vector<double>(d.rho0, d.rho0 + nvox_)));
-    if (het_c) {
-      std::vector<double> c2(nvox_);
-      for (long long i = 0; i < nvox_; ++i) c2[i] = d.c0[i] * d.c0[i];
-      upload(c0sq_, to_t<T>(c2));
+    // medium maps: T, or FP16 offsets from c0_offset / rho0_offset
+    if (maps_f16_) {
+      if (het_r) upload(rho0_, to_half(d.rho0, d.rho0_offset));
+      if (het_c) upload(c0sq_, to_half(d.c0, d.c0_offset));
+    } else {
+      if (het_r) upload(rho0_, to_t<T>(std::vector<double>(d.rho0, d.rho0 + nvox_)));
+      if (het_c) {
+        std::vector<double> c2(nvox_);
+        for (long long i = 0; i < nvox_; ++i) c2[i] = d.c0[i] * d.c0[i];
+        upload(c0sq_, to_t<T>(c2));
+      }
     }
     if (d.nonlinear && d.bona) upload(bona_, to_t<T>(std::vector<double>(d.bona, d.bona + nvox_)));
 
@@ -440,6 +447,16 @@ class Sim final : public SimBase {
     build_args(d);
     if (d.p0 && !tr_) init_p0(d);
     HIPC(hipStreamSynchronize(stream_));
+  }
+
+  // IEEE binary16 bits of (v - off), rounded to nearest even (clang _Float16)
+  std::vector<uint16_t> to_half(const double* v, double off) {
+    std::vector<uint16_t> h(nvox_);
+    for (long long i = 0; i < nvox_; ++i) {
+      const _Float16 x = (_Float16)(v[i] - off);
+      std::memcpy(&h[i], &x, sizeof(uint16_t));
+    }
+    return h;
   }
 
   PassCfg pass_cfg(long long n, bool contig, long long lines, long long outer) {
@@ -687,8 +704,11 @@ class Sim final : public SimBase {
       A.p = p_.as<T>();
       A.u = u_[a].as<T>();
       A.rho = rho_[a].as<T>();
-      A.rho0 = het_r ? rho0_.as<T>() : nullptr;
+      A.rho0 = het_r ? rho0_.p : nullptr;
       A.rho0_s = (T)d.rho0_scalar;
+      A.maps_f16 = maps_f16_ ? 1 : 0;
+      A.rho0_off = (T)d.rho0_offset;
+      A.c0_off = (T)d.c0_offset;
       A.pml = pml_[a].as<T>();
       A.pml_sg = pmlsg_[a].as<T>();
       A.Dp = Dp_[a].as<Cx<T>>();
@@ -724,7 +744,7 @@ class Sim final : public SimBase {
     X.eos = 1;
     X.rho_o1 = nd_ > 1 ? rho_[1].as<T>() : nullptr;
     X.rho_o2 = nd_ > 2 ? rho_[2].as<T>() : nullptr;
-    X.c0sq = c0sq_.p ? c0sq_.as<T>() : nullptr;
+    X.c0sq = c0sq_.p;
     X.nonlinear = d.nonlinear ? 1 : 0;
     X.bona = bona_.p ? bona_.as<T>() : nullptr;
     X.bona_s = (T)d.bona_scalar;
@@ -812,7 +832,13 @@ class Sim final : public SimBase {
     if (d.smooth_p0) smooth_p0(p0);
     // rho_d = p0 / (ndim c0^2); p = p0
     const unsigned nb = (unsigned)((nvox_ + 255) / 256);
-    HIPC(launch_init_rho<T>(nb, stream_, p0.as<T>(), c0sq_.p ? c0sq_.as<T>() : nullptr,
+    DevBuf c2tmp;
+    if (d.c0) {
+      std::vector<double> c2(nvox_);
+      for (long long i = 0; i < nvox_; ++i) c2[i] = d.c0[i] * d.c0[i];
+      upload(c2tmp, to_t<T>(c2));
+    }
+    HIPC(launch_init_rho<T>(nb, stream_, p0.as<T>(), d.c0 ? c2tmp.as<T>() : nullptr,
                             (T)(d.c0_scalar * d.c0_scalar), (T)nd_, p_.as<T>(), rho_[0].as<T>(),
                             nd_ > 1 ? rho_[1].as<T>() : nullptr, nd_ > 2 ? rho_[2].as<T>() : nullptr, nvox_));
     // u_d = -dt/(2 rho0_sg) d/dd p0
@@ -1007,7 +1033,7 @@ class Sim final : public SimBase {
   long long nvox_ = 0, NH_ = 0;
   double dt_ = 0;
   long long Nt_ = 0, Nt_alloc_ = 0, t_ = 0;
-  bool tr_ = false, stokes_ = false, finalized_ = false, cartesian_ = false;
+  bool tr_ = false, stokes_ = false, finalized_ = false, cartesian_ = false, maps_f16_ = false;
   DevBuf p_, u_[3], rho_[3], rho0_, c0sq_, bona_, S1_, S2_, rho_prev_, M1_, M2_;
   DevBuf tw_[3], Dp_[3], Dm_[3], pml_[3], pmlsg_[3];
   PassCfg cfg_[3], spec_[2], c2r_;

@@ -17,6 +17,8 @@
 // the derivative and EOS multipliers are Hermitian so the pair separates
 // exactly on the way back.
 #pragma once
+#include <hip/hip_fp16.h>
+
 #include "kw_fft.hpp"
 
 namespace kw {
@@ -40,9 +42,12 @@ struct PassArgs {
   const T* p;
   T* u;
   T* rho;
-  // medium
-  const T* rho0;  // map or null
+  // medium (maps in T, or FP16 offsets from rho0_off / c0_off when maps_f16)
+  const void* rho0;  // map or null
   T rho0_s;
+  int maps_f16;
+  T rho0_off;
+  T c0_off;
   // line tables (length N of this axis)
   const T* pml;
   const T* pml_sg;
@@ -66,7 +71,7 @@ struct PassArgs {
   int eos;             // 0 none, 1 fused r2c
   const T* rho_o1;     // other split density components (null if absent)
   const T* rho_o2;
-  const T* c0sq;       // c0^2 map or null (homogeneous: folded into M)
+  const void* c0sq;    // c0^2 map (FP16 mode: c0 - c0_off) or null (homogeneous: folded into M)
   const T* bona;       // B/A map or null
   T bona_s;            // scalar B/A (when bona == null)
   int nonlinear;
@@ -110,6 +115,26 @@ struct LineCtx {
     for (int q = 0; q < E; ++q) {
       if (va) f[ia(q)] = v[q].x;
       if (vb) f[ib(q)] = v[q].y;
+    }
+  }
+  // static medium map in T, or FP16 offset from `off` (converted to T)
+  __device__ __forceinline__ T map_at(const void* f, long long i, bool f16, T off) const {
+    return f16 ? off + (T)__half2float(reinterpret_cast<const __half*>(f)[i]) : reinterpret_cast<const T*>(f)[i];
+  }
+  __device__ __forceinline__ void load_map(const void* f, Cx<T> (&v)[E], bool f16, T off, T fill) const {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      v[q].x = va ? map_at(f, ia(q), f16, off) : fill;
+      v[q].y = vb ? map_at(f, ib(q), f16, off) : fill;
+    }
+  }
+  __device__ __forceinline__ void load_map_next(const void* f, Cx<T> (&v)[E], bool f16, T off, T fill) const {
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      int nn = t + TPL * q + 1;
+      nn = (nn == N) ? 0 : nn;
+      v[q].x = va ? map_at(f, ba + sa * (long long)nn, f16, off) : fill;
+      v[q].y = vb ? map_at(f, bb + sa * (long long)nn, f16, off) : fill;
     }
   }
   // value at element n+1 (periodic) along the line
@@ -211,8 +236,8 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
   Cx<T> r0[E];
   if (a.rho0 != nullptr) {
     Cx<T> r1[E];
-    L.load(a.rho0, r0, (T)1);
-    L.load_next(a.rho0, r1, (T)1);
+    L.load_map(a.rho0, r0, a.maps_f16 != 0, a.rho0_off, (T)1);
+    L.load_map_next(a.rho0, r1, a.maps_f16 != 0, a.rho0_off, (T)1);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       r1[q].x = (r0[q].x + r1[q].x) * (T)0.5;
@@ -350,7 +375,14 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
     }
     if (a.c0sq) {
       Cx<T> cc[E];
-      L.load(a.c0sq, cc);
+      L.load_map(a.c0sq, cc, a.maps_f16 != 0, a.c0_off, (T)0);
+      if (a.maps_f16) {
+#pragma unroll
+        for (int q = 0; q < E; ++q) {
+          cc[q].x *= cc[q].x;
+          cc[q].y *= cc[q].y;
+        }
+      }
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         f[q].x *= cc[q].x;

@@ -56,6 +56,8 @@ class SimDesc(C.Structure):
         ("pml_size", C.c_int32 * 3),
         ("pml_alpha", C.c_double * 3),
         ("c_ref", C.c_double),
+        ("c0_offset", C.c_double),
+        ("rho0_offset", C.c_double),
         ("c0", C.POINTER(C.c_double)),
         ("c0_scalar", C.c_double),
         ("rho0", C.POINTER(C.c_double)),

@@ -132,8 +132,21 @@ class Simulation:
             pass
 
 
+def quantize_map(a, storage=np.float16):
+    """FP16 offset storage of a static map: returns (dequantized Float64 map, offset).
+
+    The device stores half(a - offset) with round-to-nearest-even and
+    reconstructs offset + half in the kernel precision; the dequantized map is
+    what the kernels see, exactly.
+    """
+    a = np.asarray(a, dtype=np.float64)
+    off = float(np.round((a.min() + a.max()) / 2))
+    q = (a - off).astype(np.float16).astype(np.float64) + off
+    return q, off
+
+
 def build_desc(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSource, sensor: KWaveSensor, *,
-               pml_size=20, pml_alpha=2.0, smooth_p0: bool = True, data_cast=np.float64):
+               pml_size=20, pml_alpha=2.0, smooth_p0: bool = True, data_cast=np.float64, medium_storage=None):
     """Validate like first_order.jl:311-315/529-533 and fill kw_sim_desc.
 
     Returns (desc, keep, info); ``keep`` holds the host buffers the
@@ -155,6 +168,11 @@ def build_desc(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSourc
     Nt = int(kgrid.Nt)
     d.precision = _precision(data_cast)
     d.map_storage = 0
+    f16 = medium_storage in (np.float16, "float16", "Float16", np.dtype("float16"))
+    if medium_storage is not None and not f16:
+        raise TypeError("medium_storage must be Float16 (or None for the kernel precision)")
+    if f16:
+        d.map_storage = 2
     ps = _norm3(pml_size, nd, int, "pml_size", 0)
     pa = _norm3(pml_alpha, nd, _pml_alpha_cast, "pml_alpha", 0.0)
     for a in range(3):
@@ -169,6 +187,8 @@ def build_desc(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSourc
     else:
         if tuple(np.shape(c0)) != tuple(shape):
             raise ValueError("sound_speed array must match the grid size")
+        if f16:
+            c0, d.c0_offset = quantize_map(c0)
         cm = _colmajor(c0)
         keep.append(cm)
         d.c0 = _dptr(cm)
@@ -178,6 +198,8 @@ def build_desc(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSourc
     else:
         if tuple(np.shape(rho0)) != tuple(shape):
             raise ValueError("density array must match the grid size")
+        if f16:
+            rho0, d.rho0_offset = quantize_map(rho0)
         rm = _colmajor(rho0)
         keep.append(rm)
         d.rho0 = _dptr(rm)
@@ -295,16 +317,18 @@ def kspace_first_order(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KW
                        data_cast=np.float64, save_to_disk: Optional[str] = None, plot_sim: bool = False,
                        plot_layout: str = "default", plot_scale="auto", record_movie: Optional[str] = None,
                        progress_callback: Optional[Callable] = None, show_progress: bool = True,
-                       device: int = 0) -> SimulationOutput:
+                       medium_storage=None, device: int = 0) -> SimulationOutput:
     """Run a k-space PSTD simulation on the GPU (first_order.jl:92/287/505).
 
     ``pml_inside``, ``smooth_c0``, ``smooth_rho0``, ``plot_*`` and
     ``record_movie`` are accepted and unused, as in the reference.
+    ``medium_storage=np.float16`` (not in the reference) keeps the c0 and
+    rho0 maps in HBM as FP16 offsets; arithmetic stays in ``data_cast``.
     """
     if save_to_disk is not None:
         raise NotImplementedError("save_to_disk (HDF5 input writer) is not part of this build")
     desc, keep, info = build_desc(kgrid, medium, source, sensor, pml_size=pml_size, pml_alpha=pml_alpha,
-                                  smooth_p0=smooth_p0, data_cast=data_cast)
+                                  smooth_p0=smooth_p0, data_cast=data_cast, medium_storage=medium_storage)
     Nt = info["Nt"]
     with Simulation(desc, keep, device) as sim:
         if progress_callback is None:

@@ -132,3 +132,20 @@ def test_sensor_post_filters(angles):
     for dt, tol in ((np.float64, 1e-10), (np.float32, 1e-4)):
         out = K.kspace_first_order(copy.deepcopy(kg), med, src, sen2, data_cast=dt, **kw)
         _compare(out, ref, tol, f"filters-{angles}-{dt.__name__}")
+
+
+def test_fp16_maps():
+    """FP16 map storage: arithmetic error vs the oracle fed the dequantized maps
+    (<= 1e-4) and the total error vs the unquantized maps (reported)."""
+    print("\n[case] fp16 maps", flush=True)
+    kg, med, src, sen, kw = scenario("hetc_hetr_absorb_nonlin_psrc", (32, 24, 16), nt=40)
+    cq, _ = K.quantize_map(med.sound_speed)
+    rq, _ = K.quantize_map(med.density)
+    medq = K.KWaveMedium(sound_speed=cq, density=rq, alpha_coeff=med.alpha_coeff, alpha_power=med.alpha_power,
+                         alpha_mode=med.alpha_mode, BonA=med.BonA)
+    ref_q = O.kspace_first_order(copy.deepcopy(kg), medq, src, sen, **kw)
+    ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
+    out = K.kspace_first_order(copy.deepcopy(kg), med, src, sen, data_cast=np.float32, medium_storage=np.float16, **kw)
+    _compare(out, ref_q, 1e-4, "fp16-arith")
+    for k in out.keys():
+        print(f"    fp16-total {k:8s} rel-L2 {rel_l2(out[k], ref[k]):.3e}", flush=True)
