@@ -84,6 +84,58 @@ def cpu_baseline(n_steps_max: int = 20, budget_s: float = 20.0):
                       f"setup {stamps[0] - t_start:.1f} s excluded"}
 
 
+def bench_batch(a):
+    """Configuration 3: 64 uneven 2-D firings; batched (one launch per pass per line
+    length, hipGraph step) against the same simulations run one after another."""
+    import numpy as np
+    import kwave_amd as K
+    from kwave_amd.configs import config3
+
+    sims, kw = config3(Nt=a.warmup + a.steps)
+    descs, keep = [], []
+    for s in sims:
+        d, k, _ = K.build_desc(*s, data_cast=np.float32, **kw)
+        descs.append(d)
+        keep.append(k)
+    vox = sum(int(np.prod(s[0].grid_size())) for s in sims)
+    b = K.Batch(descs, keep, use_graph=True)
+    b.run(a.warmup)
+    b.synchronize()
+    t1 = time.perf_counter()
+    b.run(a.steps)
+    b.synchronize()
+    t2 = time.perf_counter()
+    p = b.sensor(0, "p")
+    assert np.all(np.isfinite(p))
+    b.close()
+    batched = vox * a.steps / (t2 - t1)
+    # the same simulations, one plain launch sequence each (truncated run; a rate)
+    seq_steps = max(1, min(a.steps, 200))
+    tot = 0.0
+    for d, k in zip(descs, keep):
+        s = K.Simulation(d, k)
+        s.run(a.warmup)
+        s.synchronize()
+        u1 = time.perf_counter()
+        s.run(seq_steps)
+        s.synchronize()
+        tot += time.perf_counter() - u1
+        s.close()
+    sequential = vox * seq_steps / tot
+    sizes = sorted({s[0].Nx for s in sims})
+    out = {
+        "metric": METRIC, "value": batched, "unit": "voxel-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": (t2 - t1) * 1e3 / a.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"batch of {len(sims)} 2D firings, uneven grids {sizes}, heterogeneous c0, "
+                               f"alpha0=0.5 y=1.1, linear array source/sensor, hipGraph step",
+                   "voxels_per_step": vox, "parallelism": "single GPU"},
+        "sequential_value": sequential, "batched_over_sequential": batched / sequential,
+        "sequential_sample": f"{seq_steps} steps per simulation after {a.warmup} warm-up steps",
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,11 +144,17 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", choices=["c2", "batch"], default="c2")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU slab decomposition "
               "is a design, not part of this build", file=sys.stderr)
         sys.exit(2)
+
+    if a.config == "batch":
+        if a.steps == 1200:
+            a.steps = 1500
+        return bench_batch(a)
 
     import numpy as np
     import kwave_amd as K

@@ -188,6 +188,23 @@ int32_t kw_sim_profile(kw_sim* sim, int64_t n_steps, int32_t max_classes, int32_
 
 int32_t kw_sim_destroy(kw_sim* sim);
 
+/*
+ * Batches of independent simulations (no reference counterpart: the
+ * reference runs one simulation per call).  Every pass is one launch per
+ * line length over all simulations, the step index lives on the device and,
+ * with use_graph, one step is captured in a hipGraph and replayed.  All
+ * simulations share precision, dimensionality and Nt; no time reversal.
+ * Sensor data and fields are read per simulation like kw_sim_get_*.
+ */
+typedef struct kw_batch kw_batch;
+int32_t kw_batch_create(kw_ctx* ctx, const kw_sim_desc* descs, int32_t n_sims, int32_t use_graph, kw_batch** out);
+int32_t kw_batch_run(kw_batch* batch, int64_t n_steps);
+int32_t kw_batch_synchronize(kw_batch* batch);
+int32_t kw_batch_sensor_shape(kw_batch* batch, int32_t sim, uint32_t field, int64_t* rows, int64_t* cols);
+int32_t kw_batch_get_sensor(kw_batch* batch, int32_t sim, uint32_t field, double* out);
+int32_t kw_batch_get_field(kw_batch* batch, int32_t sim, int32_t which, double* out);
+int32_t kw_batch_destroy(kw_batch* batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,10 @@ SOURCES = [
     "kw_launch_dir_f64.hip",
     "kw_launch_spec_f32.hip",
     "kw_launch_spec_f64.hip",
+    "kw_launch_c2r_f32.hip",
+    "kw_launch_c2r_f64.hip",
+    "kw_launch_batch_f32.hip",
+    "kw_launch_batch_f64.hip",
     "kw_launch_misc.hip",
     "kw_api.hip",
 ]

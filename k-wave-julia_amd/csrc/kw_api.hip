@@ -192,7 +192,9 @@ struct PassCfg {
 template <typename T>
 class Sim final : public SimBase {
  public:
-  Sim(int device, const kw_sim_desc& d) : dev_(device) { build(d); }
+  // batch_mode: fixed workgroup shapes per line length so that a batch can
+  // launch the passes of many simulations together (see Batch below)
+  Sim(int device, const kw_sim_desc& d, bool batch_mode = false) : dev_(device), batch_mode_(batch_mode) { build(d); }
   ~Sim() override {
     if (stream_) {
       (void)hipSetDevice(dev_);
@@ -333,6 +335,8 @@ class Sim final : public SimBase {
   }
 
  private:
+  template <typename U>
+  friend class Batch;
   enum Cls { C_DIR_X, C_DIR_Y, C_DIR_Z, C_SPEC_Y_FWD, C_SPEC_Z, C_SPEC_Y_INV, C_SPEC_Y, C_C2R, C_REC, C_TR, kNumClasses };
   static constexpr const char* kClassNames[kNumClasses] = {
       "k_dir_pass_x_eos", "k_dir_pass_y", "k_dir_pass_z", "k_spec_pass_y_fwd", "k_spec_pass_z_mul",
@@ -389,6 +393,8 @@ class Sim final : public SimBase {
       raise(KW_ERR_INVALID, "unknown map storage mode");
     maps_f16_ = d.map_storage == KW_MAPS_FP16;
     nvox_ = N_[0] * N_[1] * N_[2];
+    NH_ = N_[0] / 2 + 1;
+    NHP_ = (NH_ + 15) / 16 * 16;  // 128-byte aligned spectrum rows (FP32)
     dt_ = d.dt;
     Nt_ = d.Nt;
     Nt_alloc_ = d.Nt;
@@ -434,7 +440,7 @@ class Sim final : public SimBase {
     build_eos(d, het_c);
 
     // launch geometry
-    NH_ = N_[0] / 2 + 1;
+
     cfg_[0] = pass_cfg(N_[0], true, N_[1] * N_[2], 0);
     if (nd_ > 1) cfg_[1] = pass_cfg(N_[1], false, N_[0], N_[2]);
     if (nd_ > 2) cfg_[2] = pass_cfg(N_[2], false, N_[0], N_[1]);
@@ -468,7 +474,7 @@ class Sim final : public SimBase {
     while (lpw > 1 && (size_t)lpw * line_bytes > 65536) lpw /= 2;
     // pairs that exist
     const long long pairs = contig ? (lines + 1) / 2 : (lines + 1) / 2;
-    while (lpw > 1 && lpw / 2 >= pairs) lpw /= 2;
+    while (!batch_mode_ && lpw > 1 && lpw / 2 >= pairs) lpw /= 2;
     c.lpw = lpw;
     c.block = dim3(lpw * tpl);
     c.shmem = (size_t)lpw * line_bytes;
@@ -488,7 +494,7 @@ class Sim final : public SimBase {
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
     int lpw = 256 / tpl;
     while (lpw > 1 && (size_t)lpw * line_bytes > 65536) lpw /= 2;
-    while (lpw > 1 && lpw / 2 >= NH_) lpw /= 2;
+    while (!batch_mode_ && lpw > 1 && lpw / 2 >= NH_) lpw /= 2;
     c.lpw = lpw;
     c.block = dim3(lpw * tpl);
     c.shmem = (size_t)lpw * line_bytes;
@@ -540,9 +546,9 @@ class Sim final : public SimBase {
     if (stokes_) upload(M2_, m2);
     // spectra
     const long long lines = N_[1] * N_[2];
-    S1_.alloc((size_t)hx * lines * sizeof(Cx<T>));
+    S1_.alloc((size_t)NHP_ * lines * sizeof(Cx<T>));
     if (stokes_) {
-      S2_.alloc((size_t)hx * lines * sizeof(Cx<T>));
+      S2_.alloc((size_t)NHP_ * lines * sizeof(Cx<T>));
       rho_prev_.alloc(nvox_ * sizeof(T));
       HIPC(hipMemset(rho_prev_.p, 0, rho_prev_.bytes));
     }
@@ -752,6 +758,7 @@ class Sim final : public SimBase {
     X.S2 = stokes_ ? S2_.as<Cx<T>>() : nullptr;
     X.rho_prev = stokes_ ? rho_prev_.as<T>() : nullptr;
     X.inv_dt = (T)(1.0 / d.dt);
+    X.nhp = (int)NHP_;
 
     // spectral passes
     const int hy = (int)(N_[1] / 2 + 1), hz = (int)(N_[2] / 2 + 1);
@@ -766,8 +773,8 @@ class Sim final : public SimBase {
       Y.M2 = stokes_ ? M2_.as<T>() : nullptr;
       Y.NH = (int)NH_;
       Y.Sc = 1;
-      Y.Sa = NH_;
-      Y.So = NH_ * N_[1];
+      Y.Sa = NHP_;
+      Y.So = NHP_ * N_[1];
       Y.x_blocks = (int)((NH_ + spec_[0].lpw - 1) / spec_[0].lpw);
       Y.axis = 1;
       Y.Ny = (int)N_[1];
@@ -785,8 +792,8 @@ class Sim final : public SimBase {
       Z.M2 = stokes_ ? M2_.as<T>() : nullptr;
       Z.NH = (int)NH_;
       Z.Sc = 1;
-      Z.Sa = NH_ * N_[1];
-      Z.So = NH_;
+      Z.Sa = NHP_ * N_[1];
+      Z.So = NHP_;
       Z.x_blocks = (int)((NH_ + spec_[1].lpw - 1) / spec_[1].lpw);
       Z.axis = 2;
       Z.Ny = (int)N_[1];
@@ -802,6 +809,7 @@ class Sim final : public SimBase {
     ca_.tw = tw_[0].as<Cx<T>>();
     ca_.p = p_.as<T>();
     ca_.n_lines = (int)(N_[1] * N_[2]);
+    ca_.nhp = (int)NHP_;
 
     // recording
     std::memset(&ra_, 0, sizeof(ra_));
@@ -1027,10 +1035,11 @@ class Sim final : public SimBase {
   }
 
   int dev_;
+  bool batch_mode_ = false;
   hipStream_t stream_ = nullptr;
   int nd_ = 0;
   long long N_[3] = {1, 1, 1};
-  long long nvox_ = 0, NH_ = 0;
+  long long nvox_ = 0, NH_ = 0, NHP_ = 0;
   double dt_ = 0;
   long long Nt_ = 0, Nt_alloc_ = 0, t_ = 0;
   bool tr_ = false, stokes_ = false, finalized_ = false, cartesian_ = false, maps_f16_ = false;
@@ -1060,6 +1069,257 @@ class Sim final : public SimBase {
 template <typename T>
 constexpr const char* Sim<T>::kClassNames[];
 
+
+// ---------------------------------------------------------------------------
+// Batch of independent simulations: every pass is one launch per line length
+// across all simulations (workgroup -> simulation tables), the step index
+// lives on the device, and one step can be captured in a hipGraph and
+// replayed.  All simulations share precision, dimensionality and Nt.
+// ---------------------------------------------------------------------------
+struct BatchBase {
+  virtual ~BatchBase() = default;
+  virtual void run(long long n) = 0;
+  virtual void sync() = 0;
+  virtual SimBase* sim(int i) = 0;
+  virtual int size() const = 0;
+  virtual long long steps_done() const = 0;
+};
+
+template <typename T>
+class Batch final : public BatchBase {
+ public:
+  Batch(int device, const kw_sim_desc* descs, int n, bool use_graph) : dev_(device), use_graph_(use_graph) {
+    HIPC(hipSetDevice(dev_));
+    if (n < 1) raise(KW_ERR_INVALID, "empty batch");
+    for (int i = 0; i < n; ++i) {
+      if (descs[i].ndim != descs[0].ndim) raise(KW_ERR_INVALID, "a batch needs one dimensionality");
+      if (descs[i].Nt != descs[0].Nt) raise(KW_ERR_INVALID, "a batch needs one Nt");
+      if (descs[i].tr_data) raise(KW_ERR_UNSUPPORTED, "time reversal is not batched");
+      sims_.emplace_back(new Sim<T>(device, descs[i], true));
+    }
+    Nt_ = descs[0].Nt;
+    nd_ = descs[0].ndim;
+    HIPC(hipStreamCreate(&stream_));
+    std::vector<int> t0(1, 0);
+    upload(t_dev_, t0);
+    build_groups();
+    build_record();
+  }
+  ~Batch() override {
+    (void)hipSetDevice(dev_);
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+  void run(long long n) override {
+    HIPC(hipSetDevice(dev_));
+    for (long long i = 0; i < n && t_ < Nt_; ++i) {
+      if (use_graph_) {
+        if (!exec_) capture();
+        HIPC(hipGraphLaunch(exec_, stream_));
+      } else {
+        enqueue_step();
+      }
+      ++t_;
+    }
+    HIPC(hipGetLastError());
+    if (t_ == Nt_ && !finalized_) {
+      HIPC(hipStreamSynchronize(stream_));
+      for (auto& s : sims_) {
+        s->t_ = Nt_;
+        s->finalize();
+      }
+      finalized_ = true;
+    }
+  }
+  void sync() override {
+    HIPC(hipSetDevice(dev_));
+    HIPC(hipStreamSynchronize(stream_));
+    for (auto& s : sims_) s->sync();
+  }
+  SimBase* sim(int i) override {
+    if (i < 0 || i >= (int)sims_.size()) raise(KW_ERR_INVALID, "simulation index out of range");
+    sync();
+    sims_[i]->t_ = t_;
+    return sims_[i].get();
+  }
+  int size() const override { return (int)sims_.size(); }
+  long long steps_done() const override { return t_; }
+
+ private:
+  enum Kind { DIR_Y, DIR_Z, DIR_X, SPEC_Y_FWD, SPEC_Z, SPEC_Y_INV, SPEC_Y, C2R };
+  struct Group {
+    Kind kind;
+    int n;
+    dim3 block;
+    size_t shmem;
+    unsigned nwg = 0;
+    bool two = false;
+    DevBuf args, wg_sim, wg_first;
+  };
+
+  // one group per (pass kind, line length), in step order
+  void build_groups() {
+    std::vector<Kind> order;
+    if (nd_ == 3) order = {DIR_Y, DIR_Z, DIR_X, SPEC_Y_FWD, SPEC_Z, SPEC_Y_INV, C2R};
+    else if (nd_ == 2) order = {DIR_Y, DIR_X, SPEC_Y, C2R};
+    else order = {DIR_X, C2R};
+    for (Kind k : order) {
+      std::vector<int> lens;
+      for (auto& s : sims_) {
+        const int n = cfg_of(*s, k).n;
+        if (std::find(lens.begin(), lens.end(), n) == lens.end()) lens.push_back(n);
+      }
+      for (int n : lens) {
+        groups_.emplace_back(new Group);
+        Group& g = *groups_.back();
+        g.kind = k;
+        g.n = n;
+        std::vector<int> wg_sim, wg_first(sims_.size(), 0);
+        std::vector<unsigned char> args;
+        int idx = 0;
+        for (size_t i = 0; i < sims_.size(); ++i) {
+          Sim<T>& s = *sims_[i];
+          const PassCfg& c = cfg_of(s, k);
+          if (c.n != n) {
+            ++idx;
+            continue;
+          }
+          g.block = c.block;
+          g.shmem = c.shmem;
+          wg_first[i] = (int)g.nwg;
+          for (unsigned w = 0; w < c.grid.x; ++w) wg_sim.push_back((int)i);
+          g.nwg += c.grid.x;
+          g.two = g.two || s.stokes_;
+          ++idx;
+        }
+        // argument records for every simulation (indexed by simulation)
+        for (auto& sp : sims_) append_args(*sp, k, args);
+        upload(g.args, args);
+        upload(g.wg_sim, wg_sim);
+        upload(g.wg_first, wg_first);
+      }
+    }
+  }
+
+  static const PassCfg& cfg_of(const Sim<T>& s, Kind k) {
+    switch (k) {
+      case DIR_Y: return s.cfg_[1];
+      case DIR_Z: return s.cfg_[2];
+      case DIR_X: return s.cfg_[0];
+      case SPEC_Y_FWD: case SPEC_Y_INV: case SPEC_Y: return s.spec_[0];
+      case SPEC_Z: return s.spec_[1];
+      default: return s.c2r_;
+    }
+  }
+
+  void append_args(Sim<T>& s, Kind k, std::vector<unsigned char>& out) {
+    auto put = [&](const void* p, size_t n) {
+      const unsigned char* b = reinterpret_cast<const unsigned char*>(p);
+      out.insert(out.end(), b, b + n);
+    };
+    switch (k) {
+      case DIR_Y: put(&s.pa_[1], sizeof(PassArgs<T>)); break;
+      case DIR_Z: put(&s.pa_[2], sizeof(PassArgs<T>)); break;
+      case DIR_X: put(&s.pa_[0], sizeof(PassArgs<T>)); break;
+      case SPEC_Y_FWD: {
+        SpecArgs<T> a = s.sa_[0];
+        a.mode = 0;
+        put(&a, sizeof(a));
+        break;
+      }
+      case SPEC_Y_INV: {
+        SpecArgs<T> a = s.sa_[0];
+        a.mode = 1;
+        a.S2 = nullptr;
+        put(&a, sizeof(a));
+        break;
+      }
+      case SPEC_Y: {
+        SpecArgs<T> a = s.sa_[0];
+        a.mode = 2;
+        put(&a, sizeof(a));
+        break;
+      }
+      case SPEC_Z: {
+        SpecArgs<T> a = s.sa_[1];
+        a.mode = 2;
+        put(&a, sizeof(a));
+        break;
+      }
+      default: put(&s.ca_, sizeof(C2RArgs<T>)); break;
+    }
+  }
+
+  void build_record() {
+    std::vector<int> sens_sim, sens_first(sims_.size(), 0);
+    std::vector<RecArgs<T>> ra;
+    for (size_t i = 0; i < sims_.size(); ++i) {
+      Sim<T>& s = *sims_[i];
+      ra.push_back(s.ra_);
+      sens_first[i] = (int)sens_sim.size();
+      if (s.nsens_ > 0 && !s.cartesian_ && s.rec_)
+        for (long long j = 0; j < s.nsens_; ++j) sens_sim.push_back((int)i);
+    }
+    n_sens_ = (int)sens_sim.size();
+    upload(rec_args_, ra);
+    upload(sens_sim_, sens_sim);
+    upload(sens_first_, sens_first);
+  }
+
+  void enqueue_step() {
+    HIPC(launch_tick(stream_, reinterpret_cast<int*>(t_dev_.p)));
+    for (auto& gp : groups_) {
+      Group& g = *gp;
+      switch (g.kind) {
+        case DIR_Y: case DIR_Z: case DIR_X:
+          HIPC(launch_dir_batch<T>(g.n, g.kind == DIR_X, g.nwg, g.block, g.shmem, stream_, reinterpret_cast<PassArgs<T>*>(g.args.p),
+                                   reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p), reinterpret_cast<int*>(t_dev_.p)));
+          break;
+        case C2R:
+          HIPC(launch_c2r_batch<T>(g.n, g.nwg, g.block, g.shmem, stream_, reinterpret_cast<C2RArgs<T>*>(g.args.p), reinterpret_cast<int*>(g.wg_sim.p),
+                                   reinterpret_cast<int*>(g.wg_first.p)));
+          break;
+        default: {
+          const int mode = g.kind == SPEC_Y_FWD ? 0 : (g.kind == SPEC_Y_INV ? 1 : 2);
+          HIPC(launch_spec_batch<T>(g.n, mode, g.two && mode == 2, g.nwg, g.block, g.shmem, stream_,
+                                    reinterpret_cast<SpecArgs<T>*>(g.args.p), reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p)));
+        }
+      }
+    }
+    if (n_sens_ > 0)
+      HIPC(launch_record_batch<T>(stream_, reinterpret_cast<RecArgs<T>*>(rec_args_.p), reinterpret_cast<int*>(sens_sim_.p), reinterpret_cast<int*>(sens_first_.p),
+                                  n_sens_, reinterpret_cast<int*>(t_dev_.p)));
+  }
+
+  void capture() {
+    HIPC(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_step();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPC(hipStreamEndCapture(stream_, &graph_));
+    HIPC(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+  }
+
+  int dev_;
+  bool use_graph_;
+  hipStream_t stream_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  std::vector<std::unique_ptr<Sim<T>>> sims_;
+  std::vector<std::unique_ptr<Group>> groups_;
+  DevBuf t_dev_, rec_args_, sens_sim_, sens_first_;
+  int n_sens_ = 0, nd_ = 0;
+  long long Nt_ = 0, t_ = 0;
+  bool finalized_ = false;
+};
+
 }  // namespace
 
 struct kw_ctx {
@@ -1067,6 +1327,9 @@ struct kw_ctx {
 };
 struct kw_sim {
   std::unique_ptr<SimBase> impl;
+};
+struct kw_batch {
+  std::unique_ptr<BatchBase> impl;
 };
 
 #define KW_GUARD_BEGIN try {
@@ -1203,6 +1466,75 @@ int32_t kw_sim_profile(kw_sim* sim, int64_t n_steps, int32_t max_classes, int32_
 int32_t kw_sim_destroy(kw_sim* sim) {
   KW_GUARD_BEGIN
   delete sim;
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_batch_create(kw_ctx* ctx, const kw_sim_desc* descs, int32_t n_sims, int32_t use_graph, kw_batch** out) {
+  KW_GUARD_BEGIN
+  if (!ctx || !descs || !out) raise(KW_ERR_INVALID, "null argument");
+  if (n_sims < 1) raise(KW_ERR_INVALID, "empty batch");
+  for (int i = 1; i < n_sims; ++i)
+    if (descs[i].precision != descs[0].precision) raise(KW_ERR_INVALID, "a batch needs one precision");
+  std::unique_ptr<kw_batch> b(new kw_batch);
+  if (descs[0].precision == 4)
+    b->impl.reset(new Batch<float>(ctx->device, descs, n_sims, use_graph != 0));
+  else if (descs[0].precision == 8)
+    b->impl.reset(new Batch<double>(ctx->device, descs, n_sims, use_graph != 0));
+  else
+    raise(KW_ERR_INVALID, "precision must be 4 or 8");
+  *out = b.release();
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_batch_run(kw_batch* b, int64_t n_steps) {
+  KW_GUARD_BEGIN
+  if (!b) raise(KW_ERR_INVALID, "null batch");
+  if (n_steps < 0) raise(KW_ERR_INVALID, "negative step count");
+  b->impl->run(n_steps);
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_batch_synchronize(kw_batch* b) {
+  KW_GUARD_BEGIN
+  if (!b) raise(KW_ERR_INVALID, "null batch");
+  b->impl->sync();
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_batch_sensor_shape(kw_batch* b, int32_t sim, uint32_t field, int64_t* rows, int64_t* cols) {
+  KW_GUARD_BEGIN
+  if (!b || !rows || !cols) raise(KW_ERR_INVALID, "null argument");
+  long long r = 0, c = 0;
+  b->impl->sim(sim)->sensor_shape(field, &r, &c);
+  *rows = r;
+  *cols = c;
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_batch_get_sensor(kw_batch* b, int32_t sim, uint32_t field, double* out) {
+  KW_GUARD_BEGIN
+  if (!b || !out) raise(KW_ERR_INVALID, "null argument");
+  b->impl->sim(sim)->get_sensor(field, out);
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_batch_get_field(kw_batch* b, int32_t sim, int32_t which, double* out) {
+  KW_GUARD_BEGIN
+  if (!b || !out) raise(KW_ERR_INVALID, "null argument");
+  b->impl->sim(sim)->get_field(which, out);
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_batch_destroy(kw_batch* b) {
+  KW_GUARD_BEGIN
+  delete b;
   return KW_OK;
   KW_GUARD_END
 }

@@ -275,13 +275,83 @@ struct LdsLayout {
   __device__ __forceinline__ static int at(int i) { return PAD ? i + (i >> 4) : i; }
 };
 
-template <typename T, int N, int TPL, bool INV, int Ns, int R, int... Rest>
+// Twiddles of the passes after the first, W_N^((j mod Ns) r N/(Ns R)),
+// depend only on the thread's position in the line.  For small plans they
+// are loaded once per kernel into registers (all four transforms of a pass
+// kernel reuse them, conjugated for the inverse); larger plans read the
+// table, which stays L1-resident.
+template <int... Rs>
+struct RadixArray {
+  static constexpr int n = sizeof...(Rs);
+  static constexpr int r[n] = {Rs...};
+};
+template <int... Rs>
+constexpr int RadixArray<Rs...>::r[];
+
+template <int N, typename RL>
+struct TwCount;
+template <int N, int... Rs>
+struct TwCount<N, Radices<Rs...>> {
+  static constexpr int f() {
+    constexpr int rr[] = {Rs...};
+    int E = N / Plan<N>::TPL, Ns = 1, tot = 0;
+    for (int i = 0; i < (int)sizeof...(Rs); ++i) {
+      if (Ns > 1) tot += (E / rr[i]) * (rr[i] - 1);
+      Ns *= rr[i];
+    }
+    return tot;
+  }
+  static constexpr int value = f();
+};
+
+template <typename T, int N>
+struct Twiddles {
+  static constexpr int TOT = TwCount<N, typename Plan<N>::R>::value;
+  static constexpr bool REG = TOT > 0 && TOT <= 16;
+  Cx<T> w[REG ? TOT : 1];
+  const Cx<T>* g;
+};
+
+template <typename T, int N, int TPL, int Ns, int OFF, int R, int... Rest>
+struct TwLoad {
+  __device__ __forceinline__ static void run(Twiddles<T, N>& tw, const Cx<T>* __restrict__ table, int t) {
+    constexpr int NB = (N / TPL) / R;
+    if constexpr (Ns > 1) {
+#pragma unroll
+      for (int m = 0; m < NB; ++m) {
+        const int jm = (t + TPL * m) % Ns;
+#pragma unroll
+        for (int r = 1; r < R; ++r) tw.w[OFF + m * (R - 1) + (r - 1)] = table[jm * r * (N / (Ns * R))];
+      }
+    }
+    if constexpr (sizeof...(Rest) > 0)
+      TwLoad<T, N, TPL, Ns * R, (Ns > 1 ? OFF + NB * (R - 1) : OFF), Rest...>::run(tw, table, t);
+  }
+};
+template <typename T, int N, typename RL>
+struct TwLoadImpl;
+template <typename T, int N, int... Rs>
+struct TwLoadImpl<T, N, Radices<Rs...>> {
+  __device__ __forceinline__ static void run(Twiddles<T, N>& tw, const Cx<T>* __restrict__ table, int t) {
+    TwLoad<T, N, Plan<N>::TPL, 1, 0, Rs...>::run(tw, table, t);
+  }
+};
+
+// Prepare the twiddles of a line of length N for thread t of the line.
+template <typename T, int N>
+__device__ __forceinline__ Twiddles<T, N> load_twiddles(const Cx<T>* __restrict__ table, int t) {
+  Twiddles<T, N> tw;
+  tw.g = table;
+  if constexpr (Twiddles<T, N>::REG) TwLoadImpl<T, N, typename Plan<N>::R>::run(tw, table, t);
+  return tw;
+}
+
+template <typename T, int N, int TPL, bool INV, int Ns, int OFF, int R, int... Rest>
 struct StockhamPass {
   static constexpr int E = N / TPL;
   static constexpr int NB = E / R;
   static_assert(E % R == 0, "radix must divide the per-thread element count");
-  __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Cx<T>* __restrict__ tw,
-                                             int t) {
+  __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Twiddles<T, N>& tw, int t) {
     constexpr bool LAST = sizeof...(Rest) == 0;
     static_assert(!LAST || Ns * R == N, "radix product must equal N");
 #pragma unroll
@@ -294,7 +364,11 @@ struct StockhamPass {
         const int jm = j % Ns;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          Cx<T> w = tw[jm * r * (N / (Ns * R))];
+          Cx<T> w;
+          if constexpr (Twiddles<T, N>::REG)
+            w = tw.w[OFF + m * (R - 1) + (r - 1)];
+          else
+            w = tw.g[jm * r * (N / (Ns * R))];
           x[r] = INV ? cmulc(x[r], w) : cmul(x[r], w);
         }
       }
@@ -313,7 +387,7 @@ struct StockhamPass {
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = lds[LdsLayout<N>::at(t + TPL * q)];
       __syncthreads();
-      StockhamPass<T, N, TPL, INV, Ns * R, Rest...>::run(v, lds, tw, t);
+      StockhamPass<T, N, TPL, INV, Ns * R, (Ns > 1 ? OFF + NB * (R - 1) : OFF), Rest...>::run(v, lds, tw, t);
     }
   }
 };
@@ -324,9 +398,8 @@ template <typename T, int N, bool INV, int... Rs>
 struct LineFFTImpl<T, N, INV, Radices<Rs...>> {
   static constexpr int TPL = Plan<N>::TPL;
   static constexpr int E = Plan<N>::E;
-  __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Cx<T>* __restrict__ tw,
-                                             int t) {
-    StockhamPass<T, N, TPL, INV, 1, Rs...>::run(v, lds, tw, t);
+  __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Twiddles<T, N>& tw, int t) {
+    StockhamPass<T, N, TPL, INV, 1, 0, Rs...>::run(v, lds, tw, t);
   }
 };
 
@@ -334,8 +407,7 @@ struct LineFFTImpl<T, N, INV, Radices<Rs...>> {
 // line held in v (stride-TPL layout), using the line's LDS slice `lds`.
 // Every thread of the workgroup must call it (it contains barriers).
 template <typename T, int N, bool INV>
-__device__ __forceinline__ void line_fft(Cx<T> (&v)[Plan<N>::E], Cx<T>* lds, const Cx<T>* __restrict__ tw,
-                                         int t) {
+__device__ __forceinline__ void line_fft(Cx<T> (&v)[Plan<N>::E], Cx<T>* lds, const Twiddles<T, N>& tw, int t) {
   LineFFTImpl<T, N, INV, typename Plan<N>::R>::run(v, lds, tw, t);
 }
 

@@ -75,10 +75,11 @@ struct PassArgs {
   const T* bona;       // B/A map or null
   T bona_s;            // scalar B/A (when bona == null)
   int nonlinear;
-  Cx<T>* S1;           // half spectrum (Nx/2+1, Ny, Nz)
+  Cx<T>* S1;           // half spectrum (Nx/2+1 used of a row pitch nhp, Ny, Nz)
   Cx<T>* S2;           // second spectrum for the homogeneous :stokes branch
   T* rho_prev;         // previous total density (:stokes)
   T inv_dt;
+  int nhp;             // half-spectrum row pitch (Nx/2+1 rounded up to 16 complex)
 };
 
 // ---------------------------------------------------------------------------
@@ -90,36 +91,69 @@ struct PassArgs {
 // covers x in [2*LPW*xb, 2*LPW*(xb+1)) at outer index o; pair c holds x =
 // x0 + 2c (re) and x0 + 2c + 1 (im).
 
+// x / y in the kernel precision; Float32 uses the hardware reciprocal
+// (<= 1 ulp) so the division costs two instructions instead of ten.
+template <typename T>
+__device__ __forceinline__ T fdiv(T x, T y) {
+  return x / y;
+}
+template <>
+__device__ __forceinline__ float fdiv<float>(float x, float y) {
+  return x * __builtin_amdgcn_rcpf(y);
+}
+
+// Element `i` of `base` addressed as base + zero-extended 32-bit byte offset,
+// which maps to the scalar-base + vector-offset form of global loads/stores.
+template <typename U>
+__device__ __forceinline__ const U& at32(const U* base, unsigned i) {
+  return *reinterpret_cast<const U*>(reinterpret_cast<const char*>(base) + (unsigned)(i * (unsigned)sizeof(U)));
+}
+template <typename U>
+__device__ __forceinline__ U& at32(U* base, unsigned i) {
+  return *reinterpret_cast<U*>(reinterpret_cast<char*>(base) + (unsigned)(i * (unsigned)sizeof(U)));
+}
+
+// Addresses are a per-workgroup 64-bit base (uniform) plus 32-bit lane
+// offsets, so every load is a scalar-base + vector-offset access.
 template <typename T, int N>
 struct LineCtx {
   static constexpr int TPL = Plan<N>::TPL;
   static constexpr int E = Plan<N>::E;
   int t, c;
   bool va, vb;          // real line a / b valid
-  long long ba, bb;     // base offsets of lines a / b
-  long long sa;         // element stride
-  __device__ __forceinline__ long long ia(int q) const { return ba + sa * (long long)(t + TPL * q); }
-  __device__ __forceinline__ long long ib(int q) const { return bb + sa * (long long)(t + TPL * q); }
+  long long wgb;        // workgroup base element offset (uniform)
+  unsigned oa, ob;      // lane offsets of lines a / b from wgb
+  unsigned sa;          // element stride along the line
+  __device__ __forceinline__ unsigned ia(int q) const { return oa + sa * (unsigned)(t + TPL * q); }
+  __device__ __forceinline__ unsigned ib(int q) const { return ob + sa * (unsigned)(t + TPL * q); }
   __device__ __forceinline__ int n(int q) const { return t + TPL * q; }
+  __device__ __forceinline__ unsigned nx(int q) const {  // offset of element n+1 (periodic)
+    int nn = t + TPL * q + 1;
+    nn = (nn == N) ? 0 : nn;
+    return sa * (unsigned)nn;
+  }
 
   // `fill` is what an absent line of the pair holds (1 for divisors like rho0)
-  __device__ __forceinline__ void load(const T* __restrict__ f, Cx<T> (&v)[E], T fill = (T)0) const {
+  __device__ __forceinline__ void load(const T* __restrict__ f0, Cx<T> (&v)[E], T fill = (T)0) const {
+    const T* __restrict__ f = f0 + wgb;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      v[q].x = va ? f[ia(q)] : fill;
-      v[q].y = vb ? f[ib(q)] : fill;
+      v[q].x = va ? at32(f, ia(q)) : fill;
+      v[q].y = vb ? at32(f, ib(q)) : fill;
     }
   }
-  __device__ __forceinline__ void store(T* __restrict__ f, const Cx<T> (&v)[E]) const {
+  __device__ __forceinline__ void store(T* __restrict__ f0, const Cx<T> (&v)[E]) const {
+    T* __restrict__ f = f0 + wgb;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      if (va) f[ia(q)] = v[q].x;
-      if (vb) f[ib(q)] = v[q].y;
+      if (va) at32(f, ia(q)) = v[q].x;
+      if (vb) at32(f, ib(q)) = v[q].y;
     }
   }
   // static medium map in T, or FP16 offset from `off` (converted to T)
-  __device__ __forceinline__ T map_at(const void* f, long long i, bool f16, T off) const {
-    return f16 ? off + (T)__half2float(reinterpret_cast<const __half*>(f)[i]) : reinterpret_cast<const T*>(f)[i];
+  __device__ __forceinline__ T map_at(const void* f, unsigned i, bool f16, T off) const {
+    return f16 ? off + (T)__half2float(at32(reinterpret_cast<const __half*>(f) + wgb, i))
+               : at32(reinterpret_cast<const T*>(f) + wgb, i);
   }
   __device__ __forceinline__ void load_map(const void* f, Cx<T> (&v)[E], bool f16, T off, T fill) const {
 #pragma unroll
@@ -131,26 +165,14 @@ struct LineCtx {
   __device__ __forceinline__ void load_map_next(const void* f, Cx<T> (&v)[E], bool f16, T off, T fill) const {
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      int nn = t + TPL * q + 1;
-      nn = (nn == N) ? 0 : nn;
-      v[q].x = va ? map_at(f, ba + sa * (long long)nn, f16, off) : fill;
-      v[q].y = vb ? map_at(f, bb + sa * (long long)nn, f16, off) : fill;
-    }
-  }
-  // value at element n+1 (periodic) along the line
-  __device__ __forceinline__ void load_next(const T* __restrict__ f, Cx<T> (&v)[E], T fill = (T)0) const {
-#pragma unroll
-    for (int q = 0; q < E; ++q) {
-      int nn = t + TPL * q + 1;
-      nn = (nn == N) ? 0 : nn;
-      v[q].x = va ? f[ba + sa * (long long)nn] : fill;
-      v[q].y = vb ? f[bb + sa * (long long)nn] : fill;
+      v[q].x = va ? map_at(f, oa + nx(q), f16, off) : fill;
+      v[q].y = vb ? map_at(f, ob + nx(q), f16, off) : fill;
     }
   }
 };
 
 template <typename T, int N, bool CONTIG>
-__device__ __forceinline__ LineCtx<T, N> make_ctx(const PassArgs<T>& a, int& lpw_out) {
+__device__ __forceinline__ LineCtx<T, N> make_ctx(const PassArgs<T>& a, int wg, int& lpw_out) {
   constexpr int TPL = Plan<N>::TPL;
   LineCtx<T, N> L;
   const int tid = threadIdx.x;
@@ -159,24 +181,26 @@ __device__ __forceinline__ LineCtx<T, N> make_ctx(const PassArgs<T>& a, int& lpw
   if constexpr (CONTIG) {
     L.t = tid % TPL;
     L.c = tid / TPL;
-    const long long pi = (long long)blockIdx.x * lpw + L.c;
-    const long long la = 2 * pi, lb = 2 * pi + 1;
-    L.va = la < a.n_lines;
-    L.vb = lb < a.n_lines;
-    L.ba = la * (long long)N;
-    L.bb = lb * (long long)N;
+    const long long pi = (long long)wg * lpw + L.c;
+    L.va = 2 * pi < a.n_lines;
+    L.vb = 2 * pi + 1 < a.n_lines;
+    L.wgb = 2LL * wg * lpw * N;
+    L.oa = (unsigned)(2 * L.c * N);
+    L.ob = L.oa + N;
     L.sa = 1;
   } else {
     L.c = tid % lpw;
     L.t = tid / lpw;
-    const int xb = blockIdx.x % a.x_blocks;
-    const int o = blockIdx.x / a.x_blocks;
-    const int x = 2 * (xb * lpw + L.c);
+    const int xb = wg % a.x_blocks;
+    const int o = wg / a.x_blocks;
+    const int x0 = 2 * xb * lpw;
+    const int x = x0 + 2 * L.c;
     L.va = x < a.Nx;
     L.vb = x + 1 < a.Nx;
-    L.ba = (long long)x + a.So * o;
-    L.bb = L.ba + 1;
-    L.sa = a.Sa;
+    L.wgb = (long long)x0 + a.So * o;
+    L.oa = (unsigned)(2 * L.c);
+    L.ob = L.oa + 1;
+    L.sa = (unsigned)a.Sa;
   }
   return L;
 }
@@ -185,9 +209,9 @@ __device__ __forceinline__ LineCtx<T, N> make_ctx(const PassArgs<T>& a, int& lpw
 // pos packs (c << 21) | (part << 20) | n.
 template <typename T, int N>
 __device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)[Plan<N>::E], Cx<T>* lds_all,
-                                       const LineCtx<T, N>& L) {
+                                       const LineCtx<T, N>& L, int wg) {
   if (s.off == nullptr) return;
-  const int e0 = s.off[blockIdx.x], e1 = s.off[blockIdx.x + 1];
+  const int e0 = s.off[wg], e1 = s.off[wg + 1];
   if (e0 >= e1) return;  // uniform across the workgroup
   constexpr int E = Plan<N>::E;
   constexpr int P = LdsLayout<N>::PITCH;
@@ -215,22 +239,23 @@ __device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)
 // Directional pass (contiguous x lines or strided y/z lines)
 // ---------------------------------------------------------------------------
 template <typename T, int N, bool CONTIG>
-__global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
+__device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int t_index) {
   constexpr int E = Plan<N>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
   int lpw;
-  const LineCtx<T, N> L = make_ctx<T, N, CONTIG>(a, lpw);
+  const LineCtx<T, N> L = make_ctx<T, N, CONTIG>(a, wg, lpw);
   Cx<T>* lds = lds_all + L.c * P;
+  const Twiddles<T, N> tw = load_twiddles<T, N>(a.tw, L.t);
 
   // --- grad p (staggered +) : rfft -> i k e^{+ikh/2} -> irfft
   Cx<T> v[E];
   L.load(a.p, v);
-  line_fft<T, N, false>(v, lds, a.tw, L.t);
+  line_fft<T, N, false>(v, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dp[L.n(q)]);
-  line_fft<T, N, true>(v, lds, a.tw, L.t);
+  line_fft<T, N, true>(v, lds, tw, L.t);
 
   // --- velocity update  u = pml_sg (pml_sg u - dt/rho0_sg dp)
   Cx<T> r0[E];
@@ -247,8 +272,8 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         const T h = a.dt * (T)0.5;
-        v[q].x = -h / r1[q].x * v[q].x;
-        v[q].y = -h / r1[q].y * v[q].y;
+        v[q].x = -fdiv(h, r1[q].x) * v[q].x;
+        v[q].y = -fdiv(h, r1[q].y) * v[q].y;
       }
     } else {
       Cx<T> w[E];
@@ -256,8 +281,8 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         const T s = a.pml_sg[L.n(q)];
-        v[q].x = s * (s * w[q].x - a.dt / r1[q].x * v[q].x);
-        v[q].y = s * (s * w[q].y - a.dt / r1[q].y * v[q].y);
+        v[q].x = s * (s * w[q].x - fdiv(a.dt, r1[q].x) * v[q].x);
+        v[q].y = s * (s * w[q].y - fdiv(a.dt, r1[q].y) * v[q].y);
       }
     }
   } else {
@@ -281,16 +306,18 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
     L.store(a.u, v);
     return;
   }
-  inject<T, N>(a.usrc, a.t, v, lds_all, L);
+  inject<T, N>(a.usrc, t_index, v, lds_all, L, wg);
   L.store(a.u, v);
 
   // --- div u (staggered -)
-  line_fft<T, N, false>(v, lds, a.tw, L.t);
+  line_fft<T, N, false>(v, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dm[L.n(q)]);
-  line_fft<T, N, true>(v, lds, a.tw, L.t);
+  line_fft<T, N, true>(v, lds, tw, L.t);
 
-  // --- density update  rho = pml (pml rho - dt rho0 du)
+  // --- density update  rho = pml (pml rho - dt rho0 du); rho0 is re-read
+  // here (an L2 hit) rather than held in registers across the transforms
+  if (a.rho0 != nullptr) L.load_map(a.rho0, r0, a.maps_f16 != 0, a.rho0_off, (T)1);
   {
     Cx<T> w[E];
     L.load(a.rho, w);
@@ -313,7 +340,7 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = w[q];
   }
-  inject<T, N>(a.psrc, a.t, v, lds_all, L);
+  inject<T, N>(a.psrc, t_index, v, lds_all, L, wg);
   L.store(a.rho, v);
 
   if constexpr (CONTIG) {
@@ -346,10 +373,11 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
         for (int q = 0; q < E; ++q) {
           const T ba = a.bona ? bb[q].x : a.bona_s, bbv = a.bona ? bb[q].y : a.bona_s;
           const T ra = a.rho0 ? r0[q].x : a.rho0_s, rb = a.rho0 ? r0[q].y : a.rho0_s;
-          const T ea = f[q].x * ((T)1 + ba / (T)2 * f[q].x / ra);
-          const T eb = f[q].y * ((T)1 + bbv / (T)2 * f[q].y / rb);
-          const T pa = pr[q].x * ((T)1 + ba / (T)2 * pr[q].x / ra);
-          const T pb = pr[q].y * ((T)1 + bbv / (T)2 * pr[q].y / rb);
+          const T ia_ = fdiv((T)1, ra), ib_ = fdiv((T)1, rb);
+          const T ea = f[q].x * ((T)1 + ba / (T)2 * f[q].x * ia_);
+          const T eb = f[q].y * ((T)1 + bbv / (T)2 * f[q].y * ib_);
+          const T pa = pr[q].x * ((T)1 + ba / (T)2 * pr[q].x * ia_);
+          const T pb = pr[q].y * ((T)1 + bbv / (T)2 * pr[q].y * ib_);
           g[q].x = (ea - pa) * a.inv_dt;
           g[q].y = (eb - pb) * a.inv_dt;
           f[q].x = ea;
@@ -369,8 +397,8 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
       for (int q = 0; q < E; ++q) {
         const T ba = a.bona ? bb[q].x : a.bona_s, bbv = a.bona ? bb[q].y : a.bona_s;
         const T ra = a.rho0 ? r0[q].x : a.rho0_s, rb = a.rho0 ? r0[q].y : a.rho0_s;
-        f[q].x = f[q].x * ((T)1 + ba / (T)2 * f[q].x / ra);
-        f[q].y = f[q].y * ((T)1 + bbv / (T)2 * f[q].y / rb);
+        f[q].x = f[q].x * ((T)1 + ba / (T)2 * fdiv(f[q].x, ra));
+        f[q].y = f[q].y * ((T)1 + bbv / (T)2 * fdiv(f[q].y, rb));
       }
     }
     if (a.c0sq) {
@@ -399,12 +427,12 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
       Cx<T> z[E];
 #pragma unroll
       for (int q = 0; q < E; ++q) z[q] = s == 0 ? f[q] : g[q];
-      line_fft<T, N, false>(z, lds, a.tw, L.t);
+      line_fft<T, N, false>(z, lds, tw, L.t);
 #pragma unroll
       for (int q = 0; q < E; ++q) lds[LdsLayout<N>::at(L.n(q))] = z[q];
       __syncthreads();
-      const long long pi = (long long)blockIdx.x * lpw + L.c;
-      const long long oa = 2 * pi * NH, ob = (2 * pi + 1) * NH;
+      Cx<T>* __restrict__ Sw = Sdst + 2LL * wg * lpw * a.nhp;
+      const unsigned oa = (unsigned)(2 * L.c * a.nhp), ob = oa + a.nhp;
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int k = L.n(q);
@@ -414,13 +442,29 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
           const Cx<T> zm = lds[LdsLayout<N>::at(km)];
           const Cx<T> A = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
           const Cx<T> B = {(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
-          if (L.va) Sdst[oa + k] = A;
-          if (L.vb) Sdst[ob + k] = B;
+          if (L.va) at32(Sw, oa + k) = A;
+          if (L.vb) at32(Sw, ob + k) = B;
         }
       }
       __syncthreads();
     }
   }
+}
+
+template <typename T, int N, bool CONTIG>
+__global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
+  dir_pass_body<T, N, CONTIG>(a, blockIdx.x, a.t);
+}
+
+// Batched form: one launch covers the workgroups of many simulations of the
+// same line length; wg_sim maps a workgroup to its simulation, wg_first is
+// each simulation's first workgroup, *t_dev the current 1-based step.
+template <typename T, int N, bool CONTIG>
+__global__ void __launch_bounds__(256) k_dir_pass_batch(const PassArgs<T>* __restrict__ args,
+                                                        const int* __restrict__ wg_sim,
+                                                        const int* __restrict__ wg_first, const int* t_dev) {
+  const int s = wg_sim[blockIdx.x];
+  dir_pass_body<T, N, CONTIG>(args[s], blockIdx.x - wg_first[s], *t_dev);
 }
 
 // ---------------------------------------------------------------------------
@@ -446,8 +490,11 @@ struct SpecArgs {
   int MH1, MH2;       // Ny/2+1, Nz/2+1 (octant extents)
 };
 
-template <typename T, int N>
-__global__ void __launch_bounds__(256) k_spec_pass(SpecArgs<T> a) {
+// MODE: 0 forward (both spectra when S2 is set), 1 inverse, 2 fwd*M1*inv,
+// 4 fwd*(M1 S1 + M2 S2)*inv (:stokes), 3 fwd*W*inv.  Specialising on the mode
+// keeps the plain transforms at a small register footprint.
+template <typename T, int N, int MODE>
+__device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
   constexpr int TPL = Plan<N>::TPL;
   constexpr int E = Plan<N>::E;
   constexpr int P = LdsLayout<N>::PITCH;
@@ -455,79 +502,92 @@ __global__ void __launch_bounds__(256) k_spec_pass(SpecArgs<T> a) {
   Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
   const int lpw = blockDim.x / TPL;
   const int c = threadIdx.x % lpw, t = threadIdx.x / lpw;
-  const int xb = blockIdx.x % a.x_blocks;
-  const int o = blockIdx.x / a.x_blocks;
+  const int xb = wg % a.x_blocks;
+  const int o = wg / a.x_blocks;
   const int kx = xb * lpw + c;
   const bool valid = kx < a.NH;
   Cx<T>* lds = lds_all + c * P;
-  const long long base = (long long)kx * a.Sc + a.So * o;
+  const Twiddles<T, N> tw = load_twiddles<T, N>(a.tw, t);
+  const long long wgb = (long long)(xb * lpw) * a.Sc + a.So * o;   // uniform
+  const unsigned lo = (unsigned)(c * a.Sc), sa = (unsigned)a.Sa;
+  auto off = [&](int q) { return lo + sa * (unsigned)(t + TPL * q); };
+  Cx<T>* __restrict__ S1 = a.S1 + wgb;
 
   Cx<T> v[E];
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = valid ? a.S1[base + a.Sa * (t + TPL * q)] : Cx<T>{0, 0};
-  if (a.mode == 1) {
-    line_fft<T, N, true>(v, lds, a.tw, t);
-  } else if (a.mode == 3) {
-    line_fft<T, N, false>(v, lds, a.tw, t);
+  for (int q = 0; q < E; ++q) v[q] = valid ? at32(S1, off(q)) : Cx<T>{0, 0};
+  if constexpr (MODE == 1) {
+    line_fft<T, N, true>(v, lds, tw, t);
+  } else if constexpr (MODE == 3) {
+    line_fft<T, N, false>(v, lds, tw, t);
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cscale(v[q], a.W[t + TPL * q]);
-    line_fft<T, N, true>(v, lds, a.tw, t);
-  } else {
-    line_fft<T, N, false>(v, lds, a.tw, t);
-    if (a.mode == 2) {
-      Cx<T> w[E];
-      if (a.S2) {
-#pragma unroll
-        for (int q = 0; q < E; ++q) w[q] = valid ? a.S2[base + a.Sa * (t + TPL * q)] : Cx<T>{0, 0};
-        line_fft<T, N, false>(w, lds, a.tw, t);
-      }
-      // octant index of this line's other coordinate
-      int io;  // |index| of the outer coordinate
-      if (a.axis == 1) {
-        io = o;  // outer = z
-        io = (a.Nz > 1 && io > a.Nz - io) ? a.Nz - io : io;
-      } else {
-        io = o;  // outer = y
-        io = (io > a.Ny - io) ? a.Ny - io : io;
-      }
-#pragma unroll
-      for (int q = 0; q < E; ++q) {
-        int ik = t + TPL * q;
-        ik = (ik > N - ik) ? N - ik : ik;
-        long long mi;
-        if (a.axis == 1)
-          mi = (long long)kx + (long long)a.MH1 * (ik + (long long)a.MH2 * io);
-        else
-          mi = (long long)kx + (long long)a.MH1 * (io + (long long)a.MH2 * ik);
-        const T m1 = valid ? a.M1[mi] : (T)0;
-        Cx<T> r = cscale(v[q], m1);
-        if (a.S2) {
-          const T m2 = valid ? a.M2[mi] : (T)0;
-          r = cadd(r, cscale(w[q], m2));
-        }
-        v[q] = r;
-      }
-      line_fft<T, N, true>(v, lds, a.tw, t);
-    } else if (a.S2) {
-      // forward pass of the second spectrum too
+    line_fft<T, N, true>(v, lds, tw, t);
+  } else if constexpr (MODE == 0) {
+    line_fft<T, N, false>(v, lds, tw, t);
+    if (a.S2) {
+      Cx<T>* __restrict__ S2 = a.S2 + wgb;
       if (valid) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) a.S1[base + a.Sa * (t + TPL * q)] = v[q];
+        for (int q = 0; q < E; ++q) at32(S1, off(q)) = v[q];
       }
 #pragma unroll
-      for (int q = 0; q < E; ++q) v[q] = valid ? a.S2[base + a.Sa * (t + TPL * q)] : Cx<T>{0, 0};
-      line_fft<T, N, false>(v, lds, a.tw, t);
+      for (int q = 0; q < E; ++q) v[q] = valid ? at32(S2, off(q)) : Cx<T>{0, 0};
+      line_fft<T, N, false>(v, lds, tw, t);
       if (valid) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) a.S2[base + a.Sa * (t + TPL * q)] = v[q];
+        for (int q = 0; q < E; ++q) at32(S2, off(q)) = v[q];
       }
       return;
     }
+  } else {
+    line_fft<T, N, false>(v, lds, tw, t);
+    Cx<T> w[MODE == 4 ? E : 1];
+    if constexpr (MODE == 4) {
+      Cx<T>* __restrict__ S2 = a.S2 + wgb;
+#pragma unroll
+      for (int q = 0; q < E; ++q) w[q] = valid ? at32(S2, off(q)) : Cx<T>{0, 0};
+      line_fft<T, N, false>(w, lds, tw, t);
+    }
+    // octant index |i| of the outer coordinate (z for y lines, y for z lines)
+    int io = o;
+    if (a.axis == 1)
+      io = (a.Nz > 1 && io > a.Nz - io) ? a.Nz - io : io;
+    else
+      io = (io > a.Ny - io) ? a.Ny - io : io;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      int ik = t + TPL * q;
+      ik = (ik > N - ik) ? N - ik : ik;
+      const unsigned mi = a.axis == 1 ? (unsigned)(kx + a.MH1 * (ik + a.MH2 * io))
+                                      : (unsigned)(kx + a.MH1 * (io + a.MH2 * ik));
+      const T m1 = valid ? at32(a.M1, mi) : (T)0;
+      Cx<T> r = cscale(v[q], m1);
+      if constexpr (MODE == 4) {
+        const T m2 = valid ? at32(a.M2, mi) : (T)0;
+        r = cadd(r, cscale(w[q], m2));
+      }
+      v[q] = r;
+    }
+    line_fft<T, N, true>(v, lds, tw, t);
   }
   if (valid) {
 #pragma unroll
-    for (int q = 0; q < E; ++q) a.S1[base + a.Sa * (t + TPL * q)] = v[q];
+    for (int q = 0; q < E; ++q) at32(S1, off(q)) = v[q];
   }
+}
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__(256) k_spec_pass(SpecArgs<T> a) {
+  spec_pass_body<T, N, MODE>(a, blockIdx.x);
+}
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__(256) k_spec_pass_batch(const SpecArgs<T>* __restrict__ args,
+                                                         const int* __restrict__ wg_sim,
+                                                         const int* __restrict__ wg_first) {
+  const int s = wg_sim[blockIdx.x];
+  spec_pass_body<T, N, MODE>(args[s], blockIdx.x - wg_first[s]);
 }
 
 // ---------------------------------------------------------------------------
@@ -543,10 +603,11 @@ struct C2RArgs {
   const Cx<T>* tw;
   T* p;
   int n_lines;
+  int nhp;           // half-spectrum row pitch
 };
 
 template <typename T, int N>
-__global__ void __launch_bounds__(256) k_c2r_x(C2RArgs<T> a) {
+__device__ __forceinline__ void c2r_body(const C2RArgs<T>& a, int wg) {
   constexpr int TPL = Plan<N>::TPL;
   constexpr int E = Plan<N>::E;
   constexpr int P = LdsLayout<N>::PITCH;
@@ -555,26 +616,27 @@ __global__ void __launch_bounds__(256) k_c2r_x(C2RArgs<T> a) {
   Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
   const int lpw = blockDim.x / TPL;
   const int t = threadIdx.x % TPL, c = threadIdx.x / TPL;
-  const long long pi = (long long)blockIdx.x * lpw + c;
+  const long long pi = (long long)wg * lpw + c;
   const long long la = 2 * pi, lb = 2 * pi + 1;
   const bool va = la < a.n_lines, vb = lb < a.n_lines;
   Cx<T>* lds = lds_all + c * P;
+  const Twiddles<T, N> tw = load_twiddles<T, N>(a.tw, t);
   Cx<T> z[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int k = t + TPL * q;
     const bool mir = k >= NH;
     const int kk = mir ? N - k : k;
-    Cx<T> A = va ? a.S1[la * NH + kk] : Cx<T>{0, 0};
-    Cx<T> B = vb ? a.S1[lb * NH + kk] : Cx<T>{0, 0};
+    Cx<T> A = va ? a.S1[la * a.nhp + kk] : Cx<T>{0, 0};
+    Cx<T> B = vb ? a.S1[lb * a.nhp + kk] : Cx<T>{0, 0};
     if (a.M1) {
       const T m1 = a.M1[kk];
       A = cscale(A, m1);
       B = cscale(B, m1);
       if (a.S2) {
         const T m2 = a.M2[kk];
-        const Cx<T> A2 = va ? a.S2[la * NH + kk] : Cx<T>{0, 0};
-        const Cx<T> B2 = vb ? a.S2[lb * NH + kk] : Cx<T>{0, 0};
+        const Cx<T> A2 = va ? a.S2[la * a.nhp + kk] : Cx<T>{0, 0};
+        const Cx<T> B2 = vb ? a.S2[lb * a.nhp + kk] : Cx<T>{0, 0};
         A = cadd(A, cscale(A2, m2));
         B = cadd(B, cscale(B2, m2));
       }
@@ -590,13 +652,26 @@ __global__ void __launch_bounds__(256) k_c2r_x(C2RArgs<T> a) {
     }
     z[q] = Cx<T>{A.x - B.y, A.y + B.x};
   }
-  line_fft<T, N, true>(z, lds, a.tw, t);
+  line_fft<T, N, true>(z, lds, tw, t);
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int n = t + TPL * q;
     if (va) a.p[la * N + n] = z[q].x;
     if (vb) a.p[lb * N + n] = z[q].y;
   }
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_c2r_x(C2RArgs<T> a) {
+  c2r_body<T, N>(a, blockIdx.x);
+}
+
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_c2r_x_batch(const C2RArgs<T>* __restrict__ args,
+                                                     const int* __restrict__ wg_sim,
+                                                     const int* __restrict__ wg_first) {
+  const int s = wg_sim[blockIdx.x];
+  c2r_body<T, N>(args[s], blockIdx.x - wg_first[s]);
 }
 
 // ---------------------------------------------------------------------------
@@ -642,17 +717,15 @@ struct RecArgs {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_record(RecArgs<T> a) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= a.n) return;
+__device__ __forceinline__ void record_body(const RecArgs<T>& a, int j, int t_index) {
   const long long i = a.idx[j];
   const T pv = a.p[i];
-  const long long o = (long long)(a.t - 1) * a.n + j;
+  const long long o = (long long)(t_index - 1) * a.n + j;
   if (a.rec & REC_P) a.p_ts[o] = pv;
   if (a.rec & REC_P_MAX) a.p_max[j] = a.p_max[j] > pv ? a.p_max[j] : pv;
   if (a.rec & REC_P_MIN) a.p_min[j] = a.p_min[j] < pv ? a.p_min[j] : pv;
   if (a.rec & REC_P_RMS) a.p_rms[j] += (double)(pv * pv);
-  if ((a.rec & REC_P_FINAL) && a.t == a.Nt) a.p_final[j] = pv;
+  if ((a.rec & REC_P_FINAL) && t_index == a.Nt) a.p_final[j] = pv;
   T uv[3] = {0, 0, 0};
   for (int d = 0; d < a.ndim; ++d) uv[d] = a.u[d][i];
   if (a.rec & REC_UX) a.u_ts[0][o] = uv[0];
@@ -664,7 +737,7 @@ __global__ void __launch_bounds__(256) k_record(RecArgs<T> a) {
     const double mag = sqrt(m2);
     if (a.rec & REC_U_MAX) a.u_max[j] = a.u_max[j] > mag ? a.u_max[j] : mag;
     if (a.rec & REC_U_RMS) a.u_rms[j] += mag * mag;
-    if ((a.rec & REC_U_FINAL) && a.t == a.Nt) a.u_final[j] = mag;
+    if ((a.rec & REC_U_FINAL) && t_index == a.Nt) a.u_final[j] = mag;
   }
   if (a.rec & (REC_I_AVG | REC_I_MAX)) {
     double s = 0.0;
@@ -676,6 +749,32 @@ __global__ void __launch_bounds__(256) k_record(RecArgs<T> a) {
     if (a.rec & REC_I_AVG) a.I_avg[j] += inst / a.Nt;
     if (a.rec & REC_I_MAX) a.I_max[j] = a.I_max[j] > inst ? a.I_max[j] : inst;
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_record(RecArgs<T> a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < a.n) record_body<T>(a, j, a.t);
+}
+
+// Batched recording: sensor J of the concatenated lists belongs to
+// simulation sens_sim[J] at local index J - sens_first[sim].
+template <typename T>
+__global__ void __launch_bounds__(256) k_record_batch(const RecArgs<T>* __restrict__ args,
+                                                      const int* __restrict__ sens_sim,
+                                                      const int* __restrict__ sens_first, int n_total,
+                                                      const int* t_dev) {
+  const int J = blockIdx.x * blockDim.x + threadIdx.x;
+  if (J >= n_total) return;
+  const int s = sens_sim[J];
+  record_body<T>(args[s], J - sens_first[s], *t_dev);
+}
+
+// Advance the device-side step counter read by the batched kernels (one
+// lane, ordinary vector store), so a captured step graph replays unchanged.
+template <typename D>
+__global__ void k_tick(D* t_dev) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) t_dev[0] = t_dev[0] + 1;
 }
 
 // Time-reversal Dirichlet injection before a step (first_order.jl:678-689):

@@ -38,4 +38,19 @@ hipError_t launch_smooth_helpers(int which, unsigned nb, hipStream_t s, const T*
 template <typename T>
 hipError_t launch_filter(hipStream_t s, const T* p_ts, const FiltArgs& a);
 
+// Batched forms (one launch over the workgroups of many simulations).
+template <typename T>
+hipError_t launch_dir_batch(int n, bool contig, unsigned nwg, dim3 b, size_t sh, hipStream_t s, const PassArgs<T>* args,
+                            const int* wg_sim, const int* wg_first, const int* t_dev);
+template <typename T>
+hipError_t launch_spec_batch(int n, int mode, bool two, unsigned nwg, dim3 b, size_t sh, hipStream_t s,
+                             const SpecArgs<T>* args, const int* wg_sim, const int* wg_first);
+template <typename T>
+hipError_t launch_c2r_batch(int n, unsigned nwg, dim3 b, size_t sh, hipStream_t s, const C2RArgs<T>* args,
+                            const int* wg_sim, const int* wg_first);
+template <typename T>
+hipError_t launch_record_batch(hipStream_t s, const RecArgs<T>* args, const int* sens_sim, const int* sens_first,
+                               int n_total, const int* t_dev);
+hipError_t launch_tick(hipStream_t s, int* t_dev);
+
 }  // namespace kw

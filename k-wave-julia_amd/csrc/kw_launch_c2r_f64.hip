@@ -1,0 +1,19 @@
+// Complex-to-real launchers for double.
+#include "kw_launch.hpp"
+
+namespace kw {
+
+template <>
+hipError_t launch_c2r<double>(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const C2RArgs<double>& a) {
+  switch (n) {
+#define X(n_)                                                      \
+  case n_:                                                         \
+    hipLaunchKernelGGL((k_c2r_x<double, n_>), g, b, sh, s, a);        \
+    return hipGetLastError();
+    KW_SIZES(X)
+#undef X
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace kw

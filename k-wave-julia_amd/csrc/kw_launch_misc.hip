@@ -79,13 +79,26 @@ hipError_t launch_filter(hipStream_t s, const T* p_ts, const FiltArgs& a) {
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_record_batch(hipStream_t s, const RecArgs<T>* args, const int* sens_sim, const int* sens_first,
+                               int n_total, const int* t_dev) {
+  hipLaunchKernelGGL((k_record_batch<T>), dim3((unsigned)((n_total + 255) / 256)), dim3(256), 0, s, args, sens_sim,
+                     sens_first, n_total, t_dev);
+  return hipGetLastError();
+}
+hipError_t launch_tick(hipStream_t s, int* t_dev) {
+  hipLaunchKernelGGL((k_tick<int>), dim3(1), dim3(64), 0, s, t_dev);
+  return hipGetLastError();
+}
+
 #define KW_INST(T)                                                                                                    \
   template hipError_t launch_record<T>(unsigned, hipStream_t, const RecArgs<T>&);                                  \
   template hipError_t launch_tr_inject<T>(unsigned, hipStream_t, T*, T*, T*, const long long*, const T*, int);      \
   template hipError_t launch_init_rho<T>(unsigned, hipStream_t, const T*, const T*, T, T, T*, T*, T*, T*, long long); \
   template hipError_t launch_smooth_helpers<T>(int, unsigned, hipStream_t, const T*, Cx<T>*, unsigned long long*, T*, \
                                                long long);                                                          \
-  template hipError_t launch_filter<T>(hipStream_t, const T*, const FiltArgs&);
+  template hipError_t launch_filter<T>(hipStream_t, const T*, const FiltArgs&);                                  \
+  template hipError_t launch_record_batch<T>(hipStream_t, const RecArgs<T>*, const int*, const int*, int, const int*);
 KW_INST(float)
 KW_INST(double)
 #undef KW_INST

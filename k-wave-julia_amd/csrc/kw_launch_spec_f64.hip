@@ -1,28 +1,38 @@
-// Spectral-pass and complex-to-real launchers for double.
+// Spectral-pass launchers for double.
 #include "kw_launch.hpp"
 
 namespace kw {
 
-template <>
-hipError_t launch_spec<double>(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<double>& a) {
-  switch (n) {
-#define X(n_)                                                      \
-  case n_:                                                         \
-    hipLaunchKernelGGL((k_spec_pass<double, n_>), g, b, sh, s, a);    \
-    return hipGetLastError();
-    KW_SIZES(X)
-#undef X
+template <int N, typename T>
+static hipError_t spec_mode(int mode, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<T>& a) {
+  switch (mode) {
+    case 0:
+      hipLaunchKernelGGL((k_spec_pass<T, N, 0>), g, b, sh, s, a);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_spec_pass<T, N, 1>), g, b, sh, s, a);
+      break;
+    case 2:
+      if (a.S2)
+        hipLaunchKernelGGL((k_spec_pass<T, N, 4>), g, b, sh, s, a);
+      else
+        hipLaunchKernelGGL((k_spec_pass<T, N, 2>), g, b, sh, s, a);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_spec_pass<T, N, 3>), g, b, sh, s, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
   }
-  return hipErrorInvalidValue;
+  return hipGetLastError();
 }
 
 template <>
-hipError_t launch_c2r<double>(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const C2RArgs<double>& a) {
+hipError_t launch_spec<double>(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<double>& a) {
   switch (n) {
-#define X(n_)                                                      \
-  case n_:                                                         \
-    hipLaunchKernelGGL((k_c2r_x<double, n_>), g, b, sh, s, a);        \
-    return hipGetLastError();
+#define X(n_)   \
+  case n_:      \
+    return spec_mode<n_, double>(a.mode, g, b, sh, s, a);
     KW_SIZES(X)
 #undef X
   }

@@ -12,6 +12,7 @@ from .model import (AbstractKWaveGrid, Additive, AdditiveNoCorrection, Dirichlet
                     validate_record_fields)
 from .builders import make_ball, make_bowl, make_circle, make_disc, tone_burst
 from .solver import Simulation, build_desc, kspace_first_order, quantize_map
+from .batch import Batch, kspace_first_order_batch
 from ._lib import KWaveHIPError, LIB_PATH
 
 __all__ = [
@@ -20,5 +21,5 @@ __all__ = [
     "Additive", "AdditiveNoCorrection", "VALID_RECORD_FIELDS", "validate_record_fields",
     "is_homogeneous", "is_lossless", "is_nonlinear", "is_time_reversal", "has_p0",
     "has_pressure_source", "has_velocity_source", "kspace_first_order", "Simulation", "build_desc", "quantize_map",
-    "tone_burst", "make_disc", "make_circle", "make_ball", "make_bowl", "KWaveHIPError", "LIB_PATH",
+    "Batch", "kspace_first_order_batch", "tone_burst", "make_disc", "make_circle", "make_ball", "make_bowl", "KWaveHIPError", "LIB_PATH",
 ]

@@ -101,3 +101,52 @@ def config2(N: int = 256, Nt: int = 1200, pml: int = 20, amplitude: float = 1e6,
 def config2_bytes_per_voxel_update(precision: int = 4) -> int:
     """Compulsory bytes per voxel-update: 7 evolving fields read+written, 2 maps read."""
     return 2 * 7 * precision + 2 * precision
+
+
+def config3(n_sims: int = 64, Nt: int = 1500, seed: int = 42, sizes=(256, 384, 512, 768, 1024)):
+    """Batch of 2-D firings of a 128-element linear array (pitch 3, width 2 cells).
+
+    Per simulation: an N x N grid with N drawn (seeded) from `sizes`, dx = 0.1 mm;
+    c0 = 1525 + 25 z clipped to [1450, 1600] m/s where z is a standardised
+    Gaussian random field (white noise smoothed with a periodic Gaussian of
+    sigma = 10 cells = 1 mm); rho0 = 1000; alpha0 = 0.5, y = 1.1, :stokes;
+    the array sits on row x = 21 (1-based) centred in y, elements that do not
+    fit the grid are dropped; simulation i fires element i mod (elements kept)
+    with a 2.5 MHz 3-cycle tone burst; every kept array cell records p.
+    Returns a list of (kgrid, medium, source, sensor) and the shared kwargs.
+    """
+    from scipy import ndimage
+
+    rng = SplitMix64(seed)
+    sims = []
+    for i in range(n_sims):
+        N = sizes[rng.integer(0, len(sizes) - 1)]
+        noise = np.random.default_rng(seed + 1000 + i).standard_normal((N, N))
+        z = ndimage.gaussian_filter(noise, sigma=10.0, mode="wrap")
+        z = (z - z.mean()) / z.std()
+        c0 = np.clip(1525.0 + 25.0 * z, 1450.0, 1600.0)
+        kg = KWaveGrid(N, 1e-4, N, 1e-4)
+        make_time(kg, c0)
+        kg.Nt = Nt
+        kg.t_array = np.arange(Nt) * kg.dt
+        med = KWaveMedium(sound_speed=c0, density=1000.0, alpha_coeff=0.5, alpha_power=1.1, alpha_mode="stokes")
+        span = 127 * 3 + 2
+        y0 = (N - span) // 2
+        elems = []
+        for e in range(128):
+            ys = [y0 + 3 * e, y0 + 3 * e + 1]
+            if ys[0] >= 0 and ys[1] < N:
+                elems.append(ys)
+        fire = elems[i % len(elems)]
+        pm = np.zeros((N, N), dtype=bool)
+        pm[20, fire] = True
+        burst = tone_burst(1 / kg.dt, 2.5e6, 3)
+        sig = np.zeros((2, burst.size + 1))
+        sig[:, : burst.size] = burst[None, :]
+        src = KWaveSource(p_mask=pm, p=sig)
+        sm = np.zeros((N, N), dtype=bool)
+        for ys in elems:
+            sm[20, ys] = True
+        sen = KWaveSensor(mask=sm, record=["p"])
+        sims.append((kg, med, src, sen))
+    return sims, dict(pml_size=20, pml_alpha=2.0, smooth_p0=False)

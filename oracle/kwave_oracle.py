@@ -702,3 +702,147 @@ def apply_frequency_response(data, sensor, g: Grid):
     resp = np.exp(-((np.abs(freqs) - fc) ** 2) / (2 * bw ** 2))
     for j in range(n_sensor):
         P[j, :] = np.fft.ifft(np.fft.fft(P[j, :]) * resp).real
+
+
+# ---------------------------------------------------------------------------
+# Discrete adjoint of the time loop w.r.t. the sound-speed map (no reference
+# counterpart: KWave.jl has no gradient).  Scope: heterogeneous c0 map, linear
+# EOS, lossless or absorbing without dispersion (the heterogeneous branch,
+# first_order_steps.jl:273-289), additive pressure sources, p0 allowed,
+# PML and heterogeneous rho0 allowed.  Loss L = 1/2 sum_{n,j} (p^n_j - obs_{j,n})^2
+# over the recorded sensor points (n = 1..Nt).
+#
+# With D+^T = -D- (the staggered multipliers are conjugates of each other),
+# K = F^-1 kappa(1+nabla1) F symmetric, and per step n (state n-1 -> n)
+#   u_d^n   = Psg_d (Psg_d u_d^{n-1} - dt/rho0sg_d D+_d p^{n-1})
+#   rho_d^n = P_d (P_d rho_d^{n-1} - dt rho0 D-_d u_d^n) + s_n / (ndim c^2) [src]
+#   p^n     = K (c^2 sum_d rho_d^n)
+# the reverse sweep is
+#   Pbar   += M^T (p^n - obs_n)
+#   Q = K Pbar, Q' = (dK/dc_ref) Pbar ; g += 2 c rho_sum^n Q ; gref += <c^2 rho_sum^n, Q'>
+#   Rtot_d = Rbar_d + c^2 Q ; g[src] -= 2 s_n / (ndim c^3) sum_d Rtot_d[src]
+#   Utot_d = Ubar_d + D+_d (dt rho0 P_d Rtot_d) ; Rbar_d <- P_d^2 Rtot_d
+#   Pbar   = sum_d D-_d (dt/rho0sg_d Psg_d Utot_d) ; Ubar_d <- Psg_d^2 Utot_d
+# and finally g += gref / N (c_ref = mean(c0)) and, for p0 runs,
+# g -= 2 p0 / (ndim c^3) sum_d Rbar_d^0.
+# ---------------------------------------------------------------------------
+
+def _eos_multiplier(g: Grid, medium, c_ref, dt, derivative=False):
+    """kappa(1+nabla1) on the rfft grid, or its derivative w.r.t. c_ref."""
+    k = g.k
+    a = c_ref * k * dt / 2
+    with np.errstate(invalid="ignore", divide="ignore"):
+        kap = np.where(a == 0, 1.0, np.sin(a) / np.where(a == 0, 1.0, a))
+        dkap = np.where(a == 0, 0.0, (a * np.cos(a) - np.sin(a)) / np.where(a == 0, 1.0, a) ** 2 * (k * dt / 2))
+    absorb = precompute_absorption(medium, k, c_ref)
+    n1 = np.zeros_like(k)
+    dn1 = np.zeros_like(k)
+    if absorb is not None:
+        y = medium.alpha_power
+        a_np = db2neper(float(medium.alpha_coeff), y)
+        n1 = absorb["nabla1"]
+        with np.errstate(invalid="ignore", divide="ignore"):
+            dn1 = -2 * a_np * (y - 1) * c_ref ** (y - 2) * k ** y
+    m = dkap * (1 + n1) + kap * dn1 if derivative else kap * (1 + n1)
+    return m[: g.N[0] // 2 + 1]
+
+
+def grad_c0(kgrid, medium, source, sensor, p_obs, *, pml_size=20, pml_alpha=2.0, smooth_p0=False):
+    """Loss and dL/dc0 by the discrete adjoint (FP64; small grids)."""
+    g = grid_from(kgrid)
+    nd, shape, dt, Nt = g.ndim, g.N, g.dt, g.Nt
+    c0 = np.asarray(medium.sound_speed, dtype=np.float64)
+    if c0.ndim == 0:
+        raise ValueError("grad_c0 needs a sound-speed map")
+    if getattr(medium, "BonA", None) is not None:
+        raise NotImplementedError("nonlinear medium")
+    if medium.alpha_coeff is not None and medium.alpha_mode not in ("no_absorption",):
+        pass  # heterogeneous c0 never uses dispersion; absorption handled by the multiplier
+    if getattr(source, "p_mode", ADDITIVE) == DIRICHLET and getattr(source, "p", None) is not None:
+        raise NotImplementedError("Dirichlet sources")
+    rho0 = _as_param(medium.density)
+    c_ref = float(np.mean(c0))
+    psz = _norm_tuple(pml_size, nd, int)
+    pal = _norm_tuple(pml_alpha, nd, float)
+
+    def bs(v, a):
+        s = [1] * nd
+        s[a] = -1
+        return v.reshape(s)
+
+    P = [bs(get_pml(g.N[a], g.d[a], psz[a], pal[a], dt), a) for a in range(nd)]
+    Psg = [bs(get_pml(g.N[a], g.d[a], psz[a], pal[a], dt, staggered=True), a) for a in range(nd)]
+    r0sg = [rho0 if np.ndim(rho0) == 0 else stagger_density(rho0, a) for a in range(nd)]
+    M1 = _eos_multiplier(g, medium, c_ref, dt)
+    M2 = _eos_multiplier(g, medium, c_ref, dt, derivative=True)
+    K = lambda f, m: irfft_nd(m * rfft_nd(f), shape)
+    Dp = lambda f, a: spectral_gradient(f, g.k_vecs[a], g.shifts[a][0], a)
+    Dm = lambda f, a: spectral_gradient(f, g.k_vecs[a], g.shifts[a][1], a)
+    c2 = c0 ** 2
+    flat = lambda x: x.ravel(order="F")
+    sens = _findall(sensor.mask)
+    src_idx = _findall(source.p_mask) if getattr(source, "p_mask", None) is not None else np.zeros(0, np.int64)
+    sig = np.asarray(source.p, dtype=np.float64) if src_idx.size else np.zeros((0, 1))
+    if sig.ndim == 1:
+        sig = sig[:, None]
+
+    # forward, keeping rho_sum^n (small grids)
+    p = np.zeros(shape)
+    u = [np.zeros(shape) for _ in range(nd)]
+    rho = [np.zeros(shape) for _ in range(nd)]
+    p0 = getattr(source, "p0", None)
+    if p0 is not None:
+        p0 = np.asarray(p0, dtype=np.float64)
+        if smooth_p0:
+            p0 = smooth(p0, restore_max=True)
+        p = p0.copy()
+        rho = [p0 / (nd * c2) for _ in range(nd)]
+        u = [-dt / (2 * r0sg[a]) * Dp(p0, a) for a in range(nd)]
+    rsum, y = [], np.zeros((sens.size, Nt))
+    for n in range(1, Nt + 1):
+        for a in range(nd):
+            u[a] = Psg[a] * (Psg[a] * u[a] - dt / r0sg[a] * Dp(p, a))
+        for a in range(nd):
+            rho[a] = P[a] * (P[a] * rho[a] - dt * rho0 * Dm(u[a], a))
+        if src_idx.size:
+            col = min(n, sig.shape[1]) - 1
+            val = sig[:, col] / (nd * flat(c2)[src_idx])
+            for a in range(nd):
+                fa = flat(rho[a]).copy()
+                np.add.at(fa, src_idx, val)
+                rho[a] = fa.reshape(shape, order="F")
+        rs = sum(rho)
+        rsum.append(rs)
+        p = K(c2 * rs, M1)
+        y[:, n - 1] = flat(p)[sens]
+    res = y - np.asarray(p_obs, dtype=np.float64)
+    loss = 0.5 * float(np.sum(res ** 2))
+
+    grad = np.zeros(shape)
+    gref = 0.0
+    Pb = np.zeros(shape)
+    Ub = [np.zeros(shape) for _ in range(nd)]
+    Rb = [np.zeros(shape) for _ in range(nd)]
+    for n in range(Nt, 0, -1):
+        fb = flat(Pb).copy()
+        np.add.at(fb, sens, res[:, n - 1])
+        Pb = fb.reshape(shape, order="F")
+        Q = K(Pb, M1)
+        Qd = K(Pb, M2)
+        grad += 2 * c0 * rsum[n - 1] * Q
+        gref += float(np.sum(c2 * rsum[n - 1] * Qd))
+        Rt = [Rb[a] + c2 * Q for a in range(nd)]
+        if src_idx.size:
+            col = min(n, sig.shape[1]) - 1
+            tot = sum(flat(Rt[a])[src_idx] for a in range(nd))
+            gf = flat(grad).copy()
+            np.add.at(gf, src_idx, -2 * sig[:, col] / (nd * flat(c0)[src_idx] ** 3) * tot)
+            grad = gf.reshape(shape, order="F")
+        Ut = [Ub[a] + Dp(dt * rho0 * P[a] * Rt[a], a) for a in range(nd)]
+        Rb = [P[a] * P[a] * Rt[a] for a in range(nd)]
+        Pb = sum(Dm(dt / r0sg[a] * Psg[a] * Ut[a], a) for a in range(nd))
+        Ub = [Psg[a] * Psg[a] * Ut[a] for a in range(nd)]
+    if p0 is not None:
+        grad += -2 * p0 / (nd * c0 ** 3) * sum(Rb)
+    grad += gref / c0.size
+    return loss, grad
