@@ -48,6 +48,20 @@ def kernel_bytes_per_voxel(name: str, nx: int, s: int = 4) -> float:
     }.get(name, 0.0)
 
 
+def measured_traffic(kernel: str, n: int):
+    """HBM bytes per launch of `kernel` on the n^3 workload from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration),
+    or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{n}cube.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(n_steps_max: int = 20, budget_s: float = 20.0):
     """Oracle (FP32 restatement, scipy.fft threads) on the same workload, truncated."""
     import numpy as np
@@ -98,16 +112,33 @@ def bench_batch(a):
         descs.append(d)
         keep.append(k)
     vox = sum(int(np.prod(s[0].grid_size())) for s in sims)
-    b = K.Batch(descs, keep, use_graph=True)
-    b.run(a.warmup)
-    b.synchronize()
-    t1 = time.perf_counter()
-    b.run(a.steps)
-    b.synchronize()
-    t2 = time.perf_counter()
-    p = b.sensor(0, "p")
-    assert np.all(np.isfinite(p))
-    b.close()
+
+    def run_batch():
+        b = K.Batch(descs, keep, use_graph=True)
+        b.run(a.warmup)
+        b.synchronize()
+        u1 = time.perf_counter()
+        b.run(a.steps)
+        b.synchronize()
+        u2 = time.perf_counter()
+        p = b.sensor(0, "p")
+        assert np.all(np.isfinite(p))
+        b.close()
+        return u2 - u1
+
+    K.tuning.clear_table()
+    default_secs = run_batch()
+    # autotuner: one simulation of each distinct grid size (untimed)
+    tt = time.perf_counter()
+    seen = set()
+    for s in sims:
+        if s[0].Nx not in seen:
+            seen.add(s[0].Nx)
+            K.tuning.autotune(*s, steps=10, data_cast=np.float32, **kw)
+    tune_s = time.perf_counter() - tt
+    table = K.tuning.table()
+    t1 = 0.0
+    t2 = run_batch()
     batched = vox * a.steps / (t2 - t1)
     # the same simulations, one plain launch sequence each (truncated run; a rate)
     seq_steps = max(1, min(a.steps, 200))
@@ -131,9 +162,97 @@ def bench_batch(a):
                                f"alpha0=0.5 y=1.1, linear array source/sensor, hipGraph step",
                    "voxels_per_step": vox, "parallelism": "single GPU"},
         "sequential_value": sequential, "batched_over_sequential": batched / sequential,
+        "untuned_batched_value": vox * a.steps / default_secs, "autotune_s": tune_s,
+        "tuning_table": {f"{k[1]}:{k[2]}": v for k, v in sorted(table.items())},
         "sequential_sample": f"{seq_steps} steps per simulation after {a.warmup} warm-up steps",
     }
     print(json.dumps(out))
+
+
+def bench_adjoint(a):
+    """Adjoint form: 2-D 512^2 ring-array misfit gradient, FP32, checkpoints every 50 steps.
+
+    Observed data come from a forward run on the true model (untimed); the timed
+    region is gradient_c0 on the homogeneous starting model for `a.firings`
+    firings (each: forward with checkpoints, recompute, adjoint sweep).
+    """
+    import numpy as np
+    import kwave_amd as K
+    from kwave_amd.configs import config5
+
+    kg, c_true, c_start, firings, kw = config5()
+    firings = firings[: a.firings]
+    med_true = K.KWaveMedium(sound_speed=c_true, density=1000.0)
+    med0 = K.KWaveMedium(sound_speed=c_start, density=1000.0)
+    obs = [K.kspace_first_order(kg, med_true, s, r, data_cast=np.float32, **kw)["p"] for s, r in firings]
+    K.gradient_c0(kg, med0, firings[0][0], firings[0][1], obs[0], checkpoint_every=50, data_cast=np.float32, **kw)
+    t1 = time.perf_counter()
+    loss, g = K.gradient_c0(kg, med0, [f[0] for f in firings], [f[1] for f in firings], obs,
+                            checkpoint_every=50, data_cast=np.float32, **kw)
+    t2 = time.perf_counter()
+    assert np.all(np.isfinite(g))
+    vox = int(np.prod(kg.grid_size()))
+    secs = t2 - t1
+    out = {
+        "metric": METRIC, "value": vox * kg.Nt * len(firings) / secs, "unit": "voxel-updates/s", "n_gpus": 1,
+        "steps": kg.Nt * len(firings), "warmup": kg.Nt, "ms_per_step": secs * 1e3 / (kg.Nt * len(firings)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"adjoint dL/dc0, 2D {kg.Nx}^2, ring of 128 point elements, {len(firings)} firings, "
+                               f"Nt={kg.Nt}, checkpoints every 50 steps; value counts forward time steps "
+                               f"(each one forward + one recompute + one adjoint step)",
+                   "parallelism": "single GPU"},
+        "seconds_per_firing": secs / len(firings), "loss": loss,
+    }
+    print(json.dumps(out))
+
+
+def bench_c4(a):
+    """Precision form: 512^3 layered medium, FP16 map storage, FP32 arithmetic, two sensor sets."""
+    import numpy as np
+    import kwave_amd as K
+    from kwave_amd.configs import config4
+
+    kg, med, src, sensors, kw = config4(Nt=a.warmup + a.steps)
+    res = {}
+    for tag, sen in zip(("p_max_volume", "p_4096_points"), sensors):
+        desc, keep, _ = K.build_desc(kg, med, src, sen, data_cast=np.float32, **kw)
+        sim = K.Simulation(desc, keep)
+        del keep
+        sim.run(a.warmup)
+        sim.synchronize()
+        t1 = time.perf_counter()
+        sim.run(a.steps)
+        sim.synchronize()
+        t2 = time.perf_counter()
+        field = sen.record[0]
+        v = sim.sensor(field)
+        assert np.all(np.isfinite(v))
+        sim.close()
+        res[tag] = {"value": kg.Nx ** 3 * a.steps / (t2 - t1), "ms_per_step": (t2 - t1) * 1e3 / a.steps}
+    first = res["p_max_volume"]
+    out = {
+        "metric": METRIC, "value": first["value"], "unit": "voxel-updates/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": first["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"3D layered {kg.Nx}^3 dx=0.2mm PML 24, FP16 c0/rho0 maps, B/A=6, alpha0=0.6 y=1.5, "
+                               f"bowl source; sensor sets: full-volume p_max / 4096 points p",
+                   "parallelism": "single GPU"},
+        "sensor_sets": res,
+    }
+    print(json.dumps(out))
+
+
+def copy_probe():
+    """A plain elementwise copy of known size (1 GiB read + 1 GiB written per launch), for
+    calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE against algorithmic bytes."""
+    import torch
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n, "launches": 5}))
 
 
 def main():
@@ -144,17 +263,27 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--config", choices=["c2", "batch"], default="c2")
+    ap.add_argument("--config", choices=["c2", "batch", "adjoint", "c4"], default="c2")
+    ap.add_argument("--firings", type=int, default=8)
+    ap.add_argument("--copy-probe", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU slab decomposition "
               "is a design, not part of this build", file=sys.stderr)
         sys.exit(2)
 
+    if a.copy_probe:
+        return copy_probe()
     if a.config == "batch":
         if a.steps == 1200:
             a.steps = 1500
         return bench_batch(a)
+    if a.config == "adjoint":
+        return bench_adjoint(a)
+    if a.config == "c4":
+        if a.steps == 1200:
+            a.steps = 200
+        return bench_c4(a)
 
     import numpy as np
     import kwave_amd as K
@@ -175,7 +304,7 @@ def main():
     sim.synchronize()
     t2 = time.perf_counter()
     print(f"[bench] timed {a.steps} steps in {t2 - t1:.3f}s", file=sys.stderr, flush=True)
-    prof = sim.profile(a.profile_steps)
+    prof = sim.profile(a.profile_steps) if a.profile_steps > 0 else {}
     sim.synchronize()
     pmax = sim.sensor("p_max")
     if not np.all(np.isfinite(pmax)):
@@ -187,10 +316,14 @@ def main():
     secs = t2 - t1
     value = vox * a.steps / secs
     # dominant kernel = largest share of the step time
-    dom = max(prof.items(), key=lambda kv: kv[1]["ms"])
-    dname, dstat = dom
-    bpv = kernel_bytes_per_voxel(dname, N)
-    achieved = bpv * vox / (dstat["ms"] * 1e-3) / 1e9
+    roof = None
+    if prof:
+        dname, dstat = max(prof.items(), key=lambda kv: kv[1]["ms"])
+        bpv = kernel_bytes_per_voxel(dname, N)
+        achieved = bpv * vox / (dstat["ms"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dname, N),
+                "bytes_per_launch": bpv * vox, "avg_launch_ms": dstat["ms"]}
     step_b = config2_bytes_per_voxel_update(4)
     out = {
         "metric": METRIC,
@@ -209,9 +342,7 @@ def main():
                                f"8 seeded ellipsoids, alpha0=0.75 y=1.5 B/A=5, plane p source, "
                                f"z-plane sensor p/p_max/p_rms",
                    "grid": [N, N, N], "precision": "fp32", "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": bpv * vox, "avg_launch_ms": dstat["ms"]},
+        "roofline": roof,
         "step_roofline": {"bytes_per_voxel_update": step_b, "achieved": step_b * value / 1e9,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_b * value / 1e9 / HBM_PEAK_GBS},
         "kernels_ms": {k: round(v["ms"], 5) for k, v in prof.items()},

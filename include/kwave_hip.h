@@ -186,6 +186,40 @@ int32_t kw_sim_get_sensor(kw_sim* sim, uint32_t field, double* out);
 int32_t kw_sim_profile(kw_sim* sim, int64_t n_steps, int32_t max_classes, int32_t* n_classes,
                        const char** names, double* ms, int64_t* launches);
 
+/*
+ * Launch-shape autotuner (no counterpart in the reference, whose FFTW plans
+ * are tuned by FFTW.MEASURE in create_fft_plans, fft_utils.jl:44-62).  Runs
+ * timed trials of `desc` (steps timed steps each; time reversal ignored) with
+ * 1, 1/2, 1/4 and 1/8 of the default lines per workgroup for every pass and
+ * keeps the fastest per (precision, pass, line length) in a process-wide
+ * table that later kw_sim_create / kw_batch_create calls read.
+ * kw_tune_export writes the table as text lines "precision pass n lines"
+ * (*len = its length; buf gets at most cap-1 bytes plus a NUL);
+ * kw_tune_import merges such text; kw_tune_clear empties the table.
+ */
+int32_t kw_autotune(kw_ctx* ctx, const kw_sim_desc* desc, int32_t steps);
+int32_t kw_tune_export(char* buf, int64_t cap, int64_t* len);
+int32_t kw_tune_import(const char* text);
+int32_t kw_tune_clear(void);
+
+/*
+ * Misfit gradient w.r.t. the sound-speed map (no counterpart in the
+ * reference, which has no adjoint; it stands in for differentiating
+ * kspace_first_order (first_order.jl:505-728) through an AD tool).
+ * Runs the whole simulation from step 0: the forward with state checkpoints
+ * every `checkpoint_every` steps, then the discrete adjoint backwards in time,
+ * recomputing each segment from its checkpoint.  Loss L = 1/2 sum (p - p_obs)^2
+ * over the recorded sensor p (p_obs: n_sensor x Nt, column-major).  Outputs the
+ * loss and dL/dc0 over the whole grid (column-major, Float64), including the
+ * 1/N term through c_ref = mean(c0) (first_order.jl:361) and the 1/c0^2 of
+ * pressure-source injection (first_order_steps.jl:630-631).
+ * Requirements: c0 given as a map, linear medium (no B/A), additive sources,
+ * no time reversal, maps stored in the kernel precision, p recorded at a
+ * binary sensor mask.  After the call the simulation has run Nt steps and
+ * its sensor outputs are those of the forward run.
+ */
+int32_t kw_sim_grad_c0(kw_sim* sim, const double* p_obs, int64_t checkpoint_every, double* grad, double* loss);
+
 int32_t kw_sim_destroy(kw_sim* sim);
 
 /*

@@ -28,6 +28,7 @@ SOURCES = [
     "kw_launch_c2r_f64.hip",
     "kw_launch_batch_f32.hip",
     "kw_launch_batch_f64.hip",
+    "kw_launch_adj.hip",
     "kw_launch_misc.hip",
     "kw_api.hip",
 ]

@@ -3,9 +3,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -30,6 +34,23 @@ struct KwError {
     hipError_t e_ = (x);                                                                \
     if (e_ != hipSuccess) raise(KW_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// ---------------------------------------------------------------------------
+// Launch-shape tuning table: lines per workgroup for each (precision bytes,
+// pass class, line length), filled by kw_autotune's timed trials or by
+// kw_tune_import; read when a simulation or batch is created.
+// ---------------------------------------------------------------------------
+enum TuneClass { TC_DIR_X = 0, TC_DIR_Y, TC_DIR_Z, TC_SPEC_Y, TC_SPEC_Z, TC_C2R, TC_COUNT };
+std::mutex g_tune_mu;
+std::map<std::array<int, 3>, int> g_tune;
+thread_local int g_tune_div = 0;  // > 0 inside a trial: lines per workgroup = default / g_tune_div
+
+int tuned_lpw(int prec, int cls, int n, int dflt) {
+  if (g_tune_div > 0) return std::max(1, dflt / g_tune_div);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find({prec, cls, n});
+  return it == g_tune.end() ? dflt : std::min(dflt, std::max(1, it->second));
+}
 
 // ---------------------------------------------------------------------------
 // Device memory
@@ -175,6 +196,7 @@ struct SimBase {
   virtual void sensor_shape(uint32_t f, long long* r, long long* c) = 0;
   virtual void get_sensor(uint32_t f, double* out) = 0;
   virtual void profile(long long n, int maxc, int* nc, const char** names, double* ms, long long* launches) = 0;
+  virtual void grad_c0(const double* obs, long long ckpt, double* grad, double* loss) = 0;
 };
 
 struct SrcDev {
@@ -185,6 +207,7 @@ struct SrcDev {
 struct PassCfg {
   int n = 0;        // line length
   int lpw = 1;      // lines (pairs) per workgroup
+  int req = 1;      // lines per workgroup asked for (before shrinking to small grids)
   dim3 grid, block;
   size_t shmem = 0;
 };
@@ -334,7 +357,269 @@ class Sim final : public SimBase {
     if (t_ == Nt_ && !finalized_) finalize();
   }
 
+  // Autotuner trial: `warm` untimed steps, then per-class device time summed
+  // over `steps` timed steps (ms per step), with the lines per workgroup and
+  // the line length each class ran with (n = 0: class absent).
+  void trial(long long warm, long long steps, double* ms, int* lp, int* nn) {
+    HIPC(hipSetDevice(dev_));
+    for (long long i = 0; i < warm && t_ < Nt_; ++i) step(nullptr);
+    Prof pr;
+    long long done = 0;
+    for (; done < steps && t_ < Nt_; ++done) step(&pr);
+    HIPC(hipStreamSynchronize(stream_));
+    static const int cls_of[kNumClasses] = {TC_DIR_X, TC_DIR_Y,  TC_DIR_Z,  TC_SPEC_Y, TC_SPEC_Z,
+                                            TC_SPEC_Y, TC_SPEC_Y, TC_C2R,    -1,        -1};
+    for (int c = 0; c < TC_COUNT; ++c) ms[c] = 0.0;
+    for (auto& r : pr.ev) {
+      float e = 0.f;
+      HIPC(hipEventElapsedTime(&e, r.a, r.b));
+      if (cls_of[r.cls] >= 0) ms[cls_of[r.cls]] += e;
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (int c = 0; c < TC_COUNT; ++c) ms[c] /= (double)std::max<long long>(done, 1);
+    const PassCfg* cf[TC_COUNT] = {&cfg_[0], nd_ > 1 ? &cfg_[1] : nullptr, nd_ > 2 ? &cfg_[2] : nullptr,
+                                   nd_ > 1 ? &spec_[0] : nullptr, nd_ > 2 ? &spec_[1] : nullptr, &c2r_};
+    for (int c = 0; c < TC_COUNT; ++c) {
+      lp[c] = cf[c] ? cf[c]->req : 0;
+      nn[c] = cf[c] ? cf[c]->n : 0;
+    }
+  }
+
+  // Loss 1/2 sum (p - obs)^2 over the recorded sensor p and its gradient
+  // w.r.t. the c0 map by the discrete adjoint with checkpoints every `ckpt`
+  // steps (forward states saved on the way out, each segment recomputed on
+  // the way back with sum_d rho_d stored per step).  obs is n_sensor x Nt,
+  // column-major; grad is the full grid (column-major, Float64).
+  void grad_c0(const double* obs, long long ckpt, double* grad, double* loss) override {
+    HIPC(hipSetDevice(dev_));
+    if (!het_c_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient needs a sound-speed map");
+    if (nonlinear_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient supports the linear equation of state");
+    if (stokes_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient needs the heterogeneous equation-of-state branch");
+    if (maps_f16_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient needs maps in the kernel precision");
+    if (tr_ || (psrc_[0].any && p_dir_)) raise(KW_ERR_UNSUPPORTED, "time reversal / Dirichlet sources");
+    for (int a = 0; a < nd_; ++a)
+      if (usrc_[a].any && u_dir_) raise(KW_ERR_UNSUPPORTED, "Dirichlet velocity sources");
+    if (!(rec_ & KW_REC_P) || cartesian_ || nsens_ == 0) raise(KW_ERR_INVALID, "the loss needs recorded p at a binary mask");
+    if (t_ != 0) raise(KW_ERR_STATE, "grad_c0 must start from step 0");
+    if (ckpt < 1) ckpt = 1;
+    const long long Nt = Nt_, nseg = (Nt + ckpt - 1) / ckpt;
+    const size_t fb = nvox_ * sizeof(T);
+    const int nf = 1 + 2 * nd_;
+    // ---- forward with checkpoints
+    DevBuf ck;
+    ck.alloc((size_t)nseg * nf * fb);
+    auto fields = [&](int f) -> T* {
+      if (f == 0) return p_.as<T>();
+      if (f <= nd_) return u_[f - 1].as<T>();
+      return rho_[f - 1 - nd_].as<T>();
+    };
+    auto save = [&](long long s) {
+      for (int f = 0; f < nf; ++f)
+        HIPC(hipMemcpyAsync(ck.as<char>() + ((size_t)s * nf + f) * fb, fields(f), fb, hipMemcpyDeviceToDevice, stream_));
+    };
+    auto restore = [&](long long s) {
+      for (int f = 0; f < nf; ++f)
+        HIPC(hipMemcpyAsync(fields(f), ck.as<char>() + ((size_t)s * nf + f) * fb, fb, hipMemcpyDeviceToDevice, stream_));
+    };
+    for (long long s = 0; s < nseg; ++s) {
+      save(s);
+      for (long long i = 0; i < ckpt && t_ < Nt; ++i) step(nullptr);
+    }
+    HIPC(hipStreamSynchronize(stream_));
+    // ---- residuals and loss
+    const long long nres = nsens_ * Nt;
+    DevBuf dobs, dres, dloss, dg, dgref;
+    upload(dobs, std::vector<double>(obs, obs + nres));
+    dres.alloc(nres * sizeof(double));
+    const unsigned nl = adj_reduce_blocks(nres), ng = adj_reduce_blocks(nvox_);
+    upload(dloss, std::vector<double>(nl, 0.0));
+    upload(dg, std::vector<double>(nvox_, 0.0));
+    upload(dgref, std::vector<double>(ng, 0.0));
+    {
+      const void* pp[4] = {p_ts_.p, dobs.p, dres.p, dloss.p};
+      HIPC(launch_adj_small<T>(2, stream_, pp, nullptr, nres, nd_));
+    }
+    // ---- adjoint buffers
+    DevBuf Pb, Q, Qd, RS, Ub[3], Rb[3], dsig, didx, dfac;
+    Pb.alloc(fb);
+    Q.alloc(fb);
+    Qd.alloc(fb);
+    RS.alloc((size_t)ckpt * fb);
+    HIPC(hipMemsetAsync(Pb.p, 0, fb, stream_));
+    for (int a = 0; a < nd_; ++a) {
+      Ub[a].alloc(fb);
+      Rb[a].alloc(fb);
+      HIPC(hipMemsetAsync(Ub[a].p, 0, fb, stream_));
+      HIPC(hipMemsetAsync(Rb[a].p, 0, fb, stream_));
+    }
+    if (!psrc_idx_h_.empty()) {
+      upload(didx, psrc_idx_h_);
+      upload(dfac, psrc_fac_h_);
+      upload(dsig, psrc_sig_h_);   // n_src x ncols, column-major
+    }
+    // dK/dc_ref multiplier table (same octant layout as M1, 1/N folded in)
+    DevBuf M2d;
+    build_dkappa(M2d);
+    // adjoint pass arguments per axis
+    AdjArgs<T> aa[3];
+    for (int a = 0; a < nd_; ++a) {
+      AdjArgs<T>& A = aa[a];
+      std::memset(&A, 0, sizeof(A));
+      A.Rb = Rb[a].as<T>();
+      A.Ub = Ub[a].as<T>();
+      A.Pb = Pb.as<T>();
+      A.Q = Q.as<T>();
+      A.c0sq = c0sq_.as<T>();
+      A.rho0 = pa_[a].rho0;
+      A.rho0_s = pa_[a].rho0_s;
+      A.pml = pa_[a].pml;
+      A.pml_sg = pa_[a].pml_sg;
+      A.Dp = pa_[a].Dp;
+      A.Dm = pa_[a].Dm;
+      A.tw = pa_[a].tw;
+      A.dt = pa_[a].dt;
+      A.Nx = pa_[a].Nx;
+      A.Sa = pa_[a].Sa;
+      A.So = pa_[a].So;
+      A.n_lines = pa_[a].n_lines;
+      A.x_blocks = pa_[a].x_blocks;
+    }
+    // ---- reverse sweep over segments
+    for (long long s = nseg - 1; s >= 0; --s) {
+      const long long t0 = s * ckpt, t1 = std::min(Nt, t0 + ckpt);
+      restore(s);
+      t_ = t0;
+      no_record_ = true;
+      for (long long n = t0 + 1; n <= t1; ++n) {
+        pa_[0].rho_sum_out = RS.as<T>() + (size_t)(n - t0 - 1) * nvox_;
+        step(nullptr);
+      }
+      pa_[0].rho_sum_out = nullptr;
+      no_record_ = false;
+      for (long long n = t1; n > t0; --n) {
+        const T* rs = RS.as<T>() + (size_t)(n - t0 - 1) * nvox_;
+        // Pbar += M^T r_n
+        {
+          const void* pp[3] = {Pb.p, sidx_.p, dres.as<double>() + (size_t)(n - 1) * nsens_};
+          HIPC(launch_adj_small<T>(3, stream_, pp, nullptr, nsens_, nd_));
+        }
+        apply_K(Pb.as<T>(), M2d.as<T>(), Q.as<T>(), Qd.as<T>());
+        {
+          const void* pp[6] = {c0sq_.p, rs, Q.p, Qd.p, dg.p, dgref.p};
+          HIPC(launch_adj_small<T>(0, stream_, pp, nullptr, nvox_, nd_));
+        }
+        if (!psrc_idx_h_.empty()) {
+          const long long ns = (long long)psrc_idx_h_.size();
+          const long long col = std::min<long long>(n, p_nc_) - 1;
+          const void* pp[9] = {Rb[0].p, nd_ > 1 ? Rb[1].p : nullptr, nd_ > 2 ? Rb[2].p : nullptr, c0sq_.p, Q.p,
+                               didx.p, dfac.p, dsig.as<double>() + col * ns, dg.p};
+          HIPC(launch_adj_small<T>(1, stream_, pp, nullptr, ns, nd_));
+        }
+        for (int a = 0; a < nd_; ++a) {
+          aa[a].accumulate = a > 0 ? 1 : 0;
+          HIPC(launch_adj_pass<T>(cfg_[a].n, a == 0, cfg_[a].grid, cfg_[a].block, cfg_[a].shmem, stream_, aa[a]));
+        }
+      }
+    }
+    {
+      // block partial sums added in a fixed order: results do not depend on scheduling
+      std::vector<double> part(ng);
+      HIPC(hipStreamSynchronize(stream_));
+      HIPC(hipMemcpy(part.data(), dgref.p, ng * sizeof(double), hipMemcpyDeviceToHost));
+      double gsum = 0.0;
+      for (double v : part) gsum += v;
+      const double gterm = gsum / (double)nvox_;
+      const void* pp[6] = {dg.p, has_p0_ ? p0_keep_.p : nullptr, c0sq_.p, Rb[0].p,
+                           nd_ > 1 ? Rb[1].p : nullptr, nd_ > 2 ? Rb[2].p : nullptr};
+      HIPC(launch_adj_small<T>(4, stream_, pp, &gterm, nvox_, nd_));
+    }
+    HIPC(hipStreamSynchronize(stream_));
+    HIPC(hipMemcpy(grad, dg.p, nvox_ * sizeof(double), hipMemcpyDeviceToHost));
+    {
+      std::vector<double> part(nl);
+      HIPC(hipMemcpy(part.data(), dloss.p, nl * sizeof(double), hipMemcpyDeviceToHost));
+      double l = 0.0;
+      for (double v : part) l += v;
+      *loss = l;
+    }
+    t_ = Nt_;
+    if (!finalized_) finalize();
+  }
+
  private:
+  // Q = K f and Qd = dK/dc_ref f for a real field f (K = F^-1 M1 F)
+  void apply_K(const T* f, const T* M2, T* Q, T* Qd) {
+    HIPC(launch_r2c<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, f, S1_.as<Cx<T>>(), (int)(N_[1] * N_[2]),
+                       (int)NHP_, tw_[0].as<Cx<T>>()));
+    if (!S2_.p) S2_.alloc((size_t)NHP_ * N_[1] * N_[2] * sizeof(Cx<T>));
+    if (nd_ == 3) {
+      SpecArgs<T> Y = sa_[0];
+      Y.mode = 0;
+      Y.S2 = nullptr;
+      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y));
+      SpecArgs<T> Z = sa_[1];
+      Z.mode = 5;
+      Z.S2 = S2_.as<Cx<T>>();
+      Z.M2 = M2;
+      HIPC(launch_spec<T>(spec_[1].n, spec_[1].grid, spec_[1].block, spec_[1].shmem, stream_, Z));
+      Y.mode = 1;
+      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y));
+      Y.S1 = S2_.as<Cx<T>>();
+      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y));
+    } else if (nd_ == 2) {
+      SpecArgs<T> Y = sa_[0];
+      Y.mode = 5;
+      Y.S2 = S2_.as<Cx<T>>();
+      Y.M2 = M2;
+      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y));
+    }
+    C2RArgs<T> c = ca_;
+    c.S2 = nullptr;
+    c.M2 = nullptr;
+    c.p = Q;
+    c.S1 = S1_.as<Cx<T>>();
+    if (nd_ == 1) {
+      // 1-D: the multipliers are applied in the c2r pass
+      HIPC(hipMemcpyAsync(S2_.p, S1_.p, (size_t)NHP_ * sizeof(Cx<T>), hipMemcpyDeviceToDevice, stream_));
+      HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c));
+      c.S1 = S2_.as<Cx<T>>();
+      c.M1 = M2;
+      c.p = Qd;
+      HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c));
+      return;
+    }
+    HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c));
+    c.S1 = S2_.as<Cx<T>>();
+    c.p = Qd;
+    HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c));
+  }
+
+  // d/dc_ref [kappa (1 + nabla1)] / N on the octant (heterogeneous branch)
+  void build_dkappa(DevBuf& out) {
+    const long long hx = N_[0] / 2 + 1, hy = N_[1] / 2 + 1, hz = N_[2] / 2 + 1;
+    std::vector<double> kx = wavenumbers(N_[0], d_[0]);
+    std::vector<double> ky = nd_ > 1 ? wavenumbers(N_[1], d_[1]) : std::vector<double>{0.0};
+    std::vector<double> kz = nd_ > 2 ? wavenumbers(N_[2], d_[2]) : std::vector<double>{0.0};
+    std::vector<T> m(hx * hy * hz);
+    const double inv_n = 1.0 / (double)nvox_;
+    for (long long iz = 0; iz < hz; ++iz)
+      for (long long iy = 0; iy < hy; ++iy)
+        for (long long ix = 0; ix < hx; ++ix) {
+          const double kk = std::sqrt(kx[ix] * kx[ix] + ky[iy] * ky[iy] + kz[iz] * kz[iz]);
+          const double a = c_ref_ * kk * dt_ / 2;
+          const double kap = a == 0.0 ? 1.0 : std::sin(a) / a;
+          const double dkap = a == 0.0 ? 0.0 : (a * std::cos(a) - std::sin(a)) / (a * a) * (kk * dt_ / 2);
+          double n1 = 0.0, dn1 = 0.0;
+          if (absorbing_) {
+            n1 = tau_ * std::pow(kk, y_);
+            dn1 = -2.0 * anp_ * (y_ - 1.0) * std::pow(c_ref_, y_ - 2.0) * std::pow(kk, y_);
+          }
+          m[ix + hx * (iy + hy * iz)] = (T)((dkap * (1.0 + n1) + kap * dn1) * inv_n);
+        }
+    upload(out, m);
+  }
+
   template <typename U>
   friend class Batch;
   enum Cls { C_DIR_X, C_DIR_Y, C_DIR_Z, C_SPEC_Y_FWD, C_SPEC_Z, C_SPEC_Y_INV, C_SPEC_Y, C_C2R, C_REC, C_TR, kNumClasses };
@@ -402,6 +687,9 @@ class Sim final : public SimBase {
     HIPC(hipStreamCreate(&stream_));
 
     const bool het_c = d.c0 != nullptr, het_r = d.rho0 != nullptr;
+    het_c_ = het_c;
+    nonlinear_ = d.nonlinear != 0;
+    has_p0_ = d.p0 != nullptr && !tr_;
     // fields
     p_.alloc(nvox_ * sizeof(T));
     HIPC(hipMemset(p_.p, 0, p_.bytes));
@@ -441,12 +729,12 @@ class Sim final : public SimBase {
 
     // launch geometry
 
-    cfg_[0] = pass_cfg(N_[0], true, N_[1] * N_[2], 0);
-    if (nd_ > 1) cfg_[1] = pass_cfg(N_[1], false, N_[0], N_[2]);
-    if (nd_ > 2) cfg_[2] = pass_cfg(N_[2], false, N_[0], N_[1]);
-    if (nd_ > 1) spec_[0] = spec_cfg(N_[1], N_[2]);
-    if (nd_ > 2) spec_[1] = spec_cfg(N_[2], N_[1]);
-    c2r_ = pass_cfg(N_[0], true, N_[1] * N_[2], 0);
+    cfg_[0] = pass_cfg(N_[0], true, N_[1] * N_[2], 0, TC_DIR_X);
+    if (nd_ > 1) cfg_[1] = pass_cfg(N_[1], false, N_[0], N_[2], TC_DIR_Y);
+    if (nd_ > 2) cfg_[2] = pass_cfg(N_[2], false, N_[0], N_[1], TC_DIR_Z);
+    if (nd_ > 1) spec_[0] = spec_cfg(N_[1], N_[2], TC_SPEC_Y);
+    if (nd_ > 2) spec_[1] = spec_cfg(N_[2], N_[1], TC_SPEC_Z);
+    c2r_ = pass_cfg(N_[0], true, N_[1] * N_[2], 0, TC_C2R);
 
     build_sources(d, het_c);
     build_sensor(d);
@@ -465,13 +753,15 @@ class Sim final : public SimBase {
     return h;
   }
 
-  PassCfg pass_cfg(long long n, bool contig, long long lines, long long outer) {
+  PassCfg pass_cfg(long long n, bool contig, long long lines, long long outer, int cls) {
     PassCfg c;
     c.n = (int)n;
     const int tpl = plan_tpl((int)n);
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
     int lpw = 256 / tpl;
     while (lpw > 1 && (size_t)lpw * line_bytes > 65536) lpw /= 2;
+    lpw = tuned_lpw((int)sizeof(T), cls, (int)n, lpw);
+    c.req = lpw;
     // pairs that exist
     const long long pairs = contig ? (lines + 1) / 2 : (lines + 1) / 2;
     while (!batch_mode_ && lpw > 1 && lpw / 2 >= pairs) lpw /= 2;
@@ -487,13 +777,15 @@ class Sim final : public SimBase {
     }
     return c;
   }
-  PassCfg spec_cfg(long long n, long long outer) {
+  PassCfg spec_cfg(long long n, long long outer, int cls) {
     PassCfg c;
     c.n = (int)n;
     const int tpl = plan_tpl((int)n);
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
     int lpw = 256 / tpl;
     while (lpw > 1 && (size_t)lpw * line_bytes > 65536) lpw /= 2;
+    lpw = tuned_lpw((int)sizeof(T), cls, (int)n, lpw);
+    c.req = lpw;
     while (!batch_mode_ && lpw > 1 && lpw / 2 >= NH_) lpw /= 2;
     c.lpw = lpw;
     c.block = dim3(lpw * tpl);
@@ -523,6 +815,12 @@ class Sim final : public SimBase {
       eta = 2.0 * a_np * std::pow(c_ref, y - 1.0) * std::tan(M_PI * y / 2.0);
     }
     stokes_ = absorbing && !het_c && d.alpha_mode != KW_NO_DISPERSION;
+    c_ref_ = c_ref;
+    absorbing_ = absorbing;
+    tau_ = tau;
+    y_ = y;
+    anp_ = absorbing ? -tau / (2.0 * std::pow(c_ref, y - 1.0)) : 0.0;
+    for (int q = 0; q < 3; ++q) d_[q] = d.d[q];
     const double c2 = het_c ? 1.0 : d.c0_scalar * d.c0_scalar;
     std::vector<T> m1(hx * hy * hz), m2;
     if (stokes_) m2.resize(m1.size());
@@ -571,6 +869,13 @@ class Sim final : public SimBase {
       upload(pval_, vals);
       p_ns_ = ns;
       p_nc_ = nc;
+      psrc_idx_h_.assign(d.psrc_index, d.psrc_index + ns);
+      psrc_sig_h_.assign(d.psrc_signal, d.psrc_signal + ns * nc);
+      psrc_fac_h_.resize(ns);
+      for (long long j = 0; j < ns; ++j) {
+        const double c = het_c ? d.c0[d.psrc_index[j]] : d.c0_scalar;
+        psrc_fac_h_[j] = -2.0 / ((double)nd_ * c * c * c);   // d/dc of 1/(ndim c^2)
+      }
       p_dir_ = d.p_mode == KW_DIRICHLET;
       for (int a = 0; a < nd_; ++a) build_csr(a, d.psrc_index, ns, psrc_[a]);
     }
@@ -835,7 +1140,7 @@ class Sim final : public SimBase {
 
   // initialize_p0_{1,2,3}d! (first_order_steps.jl:58-191) with p0 already smoothed
   void init_p0(const kw_sim_desc& d) {
-    DevBuf p0;
+    DevBuf& p0 = p0_keep_;
     upload(p0, to_t<T>(std::vector<double>(d.p0, d.p0 + nvox_)));
     if (d.smooth_p0) smooth_p0(p0);
     // rho_d = p0 / (ndim c0^2); p = p0
@@ -964,7 +1269,7 @@ class Sim final : public SimBase {
       launch(pr, C_SPEC_Y, [&] { HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y)); });
     }
     launch(pr, C_C2R, [&] { HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, ca_)); });
-    if (nsens_ > 0 && !cartesian_ && rec_) {
+    if (nsens_ > 0 && !cartesian_ && rec_ && !no_record_) {
       ra_.t = t;
       launch(pr, C_REC, [&] {
         HIPC(launch_record<T>((unsigned)((nsens_ + 255) / 256), stream_, ra_));
@@ -1060,6 +1365,12 @@ class Sim final : public SimBase {
   uint32_t rec_ = 0;
   DevBuf sidx_, p_ts_, u_ts_[3], p_max_, p_min_, p_rms_, p_final_, u_max_, u_rms_, u_final_, I_avg_, I_max_;
   DevBuf tr_vals_;
+  // adjoint
+  bool het_c_ = false, nonlinear_ = false, has_p0_ = false, absorbing_ = false, no_record_ = false;
+  double c_ref_ = 0, tau_ = 0, y_ = 0, anp_ = 0, d_[3] = {1, 1, 1};
+  std::vector<long long> psrc_idx_h_;
+  std::vector<double> psrc_sig_h_, psrc_fac_h_;
+  DevBuf p0_keep_;
   // post-filters
   bool dir_ = false, fr_ = false, filtered_ = false;
   double dir_size_ = 0, dir_ang_s_ = 0, fc_ = 0, bw_ = 1, dmin_ = 1;
@@ -1186,6 +1497,8 @@ class Batch final : public BatchBase {
             ++idx;
             continue;
           }
+          if (g.nwg > 0 && (c.block.x != g.block.x || c.shmem != g.shmem))
+            raise(KW_ERR_STATE, "batched simulations disagree on the workgroup shape of a line length");
           g.block = c.block;
           g.shmem = c.shmem;
           wg_first[i] = (int)g.nwg;
@@ -1319,6 +1632,50 @@ class Batch final : public BatchBase {
   long long Nt_ = 0, t_ = 0;
   bool finalized_ = false;
 };
+
+// Timed trials of the lines-per-workgroup choices {default, /2, /4, /8} for
+// every pass class of one simulation; the fastest per class goes into the
+// tuning table.  Each trial builds and runs its own copy of the simulation.
+template <typename T>
+void autotune(int dev, const kw_sim_desc& d0, int steps) {
+  kw_sim_desc d = d0;
+  d.tr_data = nullptr;
+  const long long warm = 2;
+  d.Nt = warm + steps;
+  double best_ms[TC_COUNT];
+  int best_lp[TC_COUNT], best_n[TC_COUNT] = {0};
+  for (int c = 0; c < TC_COUNT; ++c) best_ms[c] = INFINITY;
+  int prev_lp[TC_COUNT] = {0};
+  for (int div = 1; div <= 8; div *= 2) {
+    std::unique_ptr<Sim<T>> s;
+    g_tune_div = div;
+    try {
+      s.reset(new Sim<T>(dev, d, false));
+    } catch (...) {
+      g_tune_div = 0;
+      throw;
+    }
+    g_tune_div = 0;
+    double ms[TC_COUNT];
+    int lp[TC_COUNT], nn[TC_COUNT];
+    s->trial(warm, steps, ms, lp, nn);
+    bool changed = false;
+    for (int c = 0; c < TC_COUNT; ++c) {
+      if (nn[c] == 0) continue;
+      if (lp[c] != prev_lp[c]) changed = true;
+      prev_lp[c] = lp[c];
+      if (ms[c] < best_ms[c]) {
+        best_ms[c] = ms[c];
+        best_lp[c] = lp[c];
+        best_n[c] = nn[c];
+      }
+    }
+    if (!changed) break;  // every class already at one line per workgroup
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (int c = 0; c < TC_COUNT; ++c)
+    if (best_n[c] > 0) g_tune[{(int)sizeof(T), c, best_n[c]}] = best_lp[c];
+}
 
 }  // namespace
 
@@ -1459,6 +1816,74 @@ int32_t kw_sim_profile(kw_sim* sim, int64_t n_steps, int32_t max_classes, int32_
   sim->impl->profile(n_steps, max_classes, &nc, names, ms, l.data());
   for (int i = 0; i < nc; ++i) launches[i] = l[i];
   *n_classes = nc;
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_autotune(kw_ctx* ctx, const kw_sim_desc* desc, int32_t steps) {
+  KW_GUARD_BEGIN
+  if (!ctx || !desc) raise(KW_ERR_INVALID, "null argument");
+  if (steps < 1) steps = 1;
+  HIPC(hipSetDevice(ctx->device));
+  if (desc->precision == 4)
+    autotune<float>(ctx->device, *desc, steps);
+  else if (desc->precision == 8)
+    autotune<double>(ctx->device, *desc, steps);
+  else
+    raise(KW_ERR_INVALID, "precision must be 4 or 8");
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_tune_export(char* buf, int64_t cap, int64_t* len) {
+  KW_GUARD_BEGIN
+  if (!len) raise(KW_ERR_INVALID, "null argument");
+  std::ostringstream os;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    for (auto& kv : g_tune) os << kv.first[0] << ' ' << kv.first[1] << ' ' << kv.first[2] << ' ' << kv.second << '\n';
+  }
+  const std::string s = os.str();
+  *len = (int64_t)s.size();
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>(s.size(), (size_t)cap - 1);
+    std::memcpy(buf, s.data(), n);
+    buf[n] = '\0';
+  }
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_tune_import(const char* text) {
+  KW_GUARD_BEGIN
+  if (!text) raise(KW_ERR_INVALID, "null argument");
+  std::istringstream is(text);
+  std::map<std::array<int, 3>, int> add;
+  int p, c, n, l;
+  while (is >> p >> c >> n >> l) {
+    if ((p != 4 && p != 8) || c < 0 || c >= TC_COUNT || !size_supported(n) || l < 1 || l > 256)
+      raise(KW_ERR_INVALID, "malformed tuning entry");
+    add[{p, c, n}] = l;
+  }
+  if (!is.eof()) raise(KW_ERR_INVALID, "malformed tuning text");
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (auto& kv : add) g_tune[kv.first] = kv.second;
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_tune_clear(void) {
+  KW_GUARD_BEGIN
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_sim_grad_c0(kw_sim* sim, const double* p_obs, int64_t checkpoint_every, double* grad, double* loss) {
+  KW_GUARD_BEGIN
+  if (!sim || !p_obs || !grad || !loss) raise(KW_ERR_INVALID, "null argument");
+  sim->impl->grad_c0(p_obs, checkpoint_every, grad, loss);
   return KW_OK;
   KW_GUARD_END
 }

@@ -80,6 +80,7 @@ struct PassArgs {
   T* rho_prev;         // previous total density (:stokes)
   T inv_dt;
   int nhp;             // half-spectrum row pitch (Nx/2+1 rounded up to 16 complex)
+  T* rho_sum_out;      // if set, the x pass also stores sum_d rho_d (adjoint recompute)
 };
 
 // ---------------------------------------------------------------------------
@@ -361,6 +362,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
 #pragma unroll
       for (int q = 0; q < E; ++q) f[q] = cadd(f[q], w[q]);
     }
+    if (a.rho_sum_out) L.store(a.rho_sum_out, f);
     Cx<T> g[E];  // second spectrum input (:stokes)
     if (a.S2 != nullptr) {
       Cx<T> pr[E];
@@ -491,7 +493,8 @@ struct SpecArgs {
 };
 
 // MODE: 0 forward (both spectra when S2 is set), 1 inverse, 2 fwd*M1*inv,
-// 4 fwd*(M1 S1 + M2 S2)*inv (:stokes), 3 fwd*W*inv.  Specialising on the mode
+// 4 fwd*(M1 S1 + M2 S2)*inv (:stokes), 3 fwd*W*inv, 5 fwd then S1 = inv(M1 .),
+// S2 = inv(M2 .) (the adjoint's K and dK/dc_ref applied to one spectrum).  Specialising on the mode
 // keeps the plain transforms at a small register footprint.
 template <typename T, int N, int MODE>
 __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
@@ -540,13 +543,47 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
       }
       return;
     }
+  } else if constexpr (MODE == 5) {
+    line_fft<T, N, false>(v, lds, tw, t);
+    int io = o;
+    if (a.axis == 1)
+      io = (a.Nz > 1 && io > a.Nz - io) ? a.Nz - io : io;
+    else
+      io = (io > a.Ny - io) ? a.Ny - io : io;
+    Cx<T> w[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      int ik = t + TPL * q;
+      ik = (ik > N - ik) ? N - ik : ik;
+      const unsigned mi = a.axis == 1 ? (unsigned)(kx + a.MH1 * (ik + a.MH2 * io))
+                                      : (unsigned)(kx + a.MH1 * (io + a.MH2 * ik));
+      const T m1 = valid ? at32(a.M1, mi) : (T)0;
+      const T m2 = valid ? at32(a.M2, mi) : (T)0;
+      w[q] = cscale(v[q], m2);
+      v[q] = cscale(v[q], m1);
+    }
+    line_fft<T, N, true>(w, lds, tw, t);
+    if (valid) {
+      Cx<T>* __restrict__ S2 = a.S2 + wgb;
+#pragma unroll
+      for (int q = 0; q < E; ++q) at32(S2, off(q)) = w[q];
+    }
+    line_fft<T, N, true>(v, lds, tw, t);
   } else {
     line_fft<T, N, false>(v, lds, tw, t);
     Cx<T> w[MODE == 4 ? E : 1];
+    // In a batch, a MODE 4 launch also covers simulations without a second
+    // spectrum (S2 and M2 null): those take w = 0 and never touch S2/M2.
+    const bool has2 = MODE == 4 && a.S2 != nullptr;   // uniform per workgroup
     if constexpr (MODE == 4) {
-      Cx<T>* __restrict__ S2 = a.S2 + wgb;
+      if (has2) {
+        const Cx<T>* __restrict__ S2 = a.S2 + wgb;
 #pragma unroll
-      for (int q = 0; q < E; ++q) w[q] = valid ? at32(S2, off(q)) : Cx<T>{0, 0};
+        for (int q = 0; q < E; ++q) w[q] = valid ? at32(S2, off(q)) : Cx<T>{0, 0};
+      } else {
+#pragma unroll
+        for (int q = 0; q < E; ++q) w[q] = Cx<T>{0, 0};
+      }
       line_fft<T, N, false>(w, lds, tw, t);
     }
     // octant index |i| of the outer coordinate (z for y lines, y for z lines)
@@ -564,7 +601,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
       const T m1 = valid ? at32(a.M1, mi) : (T)0;
       Cx<T> r = cscale(v[q], m1);
       if constexpr (MODE == 4) {
-        const T m2 = valid ? at32(a.M2, mi) : (T)0;
+        const T m2 = (valid && has2) ? at32(a.M2, mi) : (T)0;
         r = cadd(r, cscale(w[q], m2));
       }
       v[q] = r;
@@ -925,6 +962,236 @@ __global__ void __launch_bounds__(256) k_filter_inv(FiltArgs a) {
     }
     a.out[(long long)n * a.ns + j] = acc / a.nt;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Discrete adjoint of the time step w.r.t. the sound-speed map.
+// Per reverse step n (see the oracle's grad_c0 for the derivation):
+//   Pbar += M^T r_n ; Q = K Pbar, Q' = dK/dc_ref Pbar   (k_r2c_x, k_spec_pass 5, k_c2r_x)
+//   g += 2 c rho_sum^n Q ; gref += <c^2 rho_sum^n, Q'>   (k_adj_grad)
+//   per axis d (k_adj_pass): Rt = Rbar_d + c^2 Q ; Rbar_d <- P^2 Rt ;
+//     Ut = Ubar_d + D+ (dt rho0 P Rt) ; Ubar_d <- Psg^2 Ut ;
+//     Pbar (+)= D- (dt/rho0sg Psg Ut)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct AdjArgs {
+  T* Rb;            // adjoint of rho_d
+  T* Ub;            // adjoint of u_d
+  T* Pb;            // adjoint of p (written by the first axis, accumulated by the others)
+  const T* Q;       // K Pbar
+  const T* c0sq;    // c0^2 map
+  const void* rho0; // map (T) or null
+  T rho0_s;
+  const T* pml;
+  const T* pml_sg;
+  const Cx<T>* Dp;
+  const Cx<T>* Dm;
+  const Cx<T>* tw;
+  T dt;
+  int Nx;
+  long long Sa, So;
+  int n_lines, x_blocks;
+  int accumulate;   // 0: Pb = contribution, 1: Pb += contribution
+};
+
+template <typename T, int N, bool CONTIG>
+__global__ void __launch_bounds__(256) k_adj_pass(AdjArgs<T> a) {
+  constexpr int E = Plan<N>::E;
+  constexpr int P = LdsLayout<N>::PITCH;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
+  PassArgs<T> geo;
+  geo.Nx = a.Nx;
+  geo.Sa = a.Sa;
+  geo.So = a.So;
+  geo.n_lines = a.n_lines;
+  geo.x_blocks = a.x_blocks;
+  int lpw;
+  const LineCtx<T, N> L = make_ctx<T, N, CONTIG>(geo, blockIdx.x, lpw);
+  Cx<T>* lds = lds_all + L.c * P;
+  const Twiddles<T, N> tw = load_twiddles<T, N>(a.tw, L.t);
+
+  Cx<T> rt[E], cc[E], v[E];
+  L.load(a.Rb, rt);
+  L.load(a.Q, v);
+  L.load(a.c0sq, cc);
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const T s = a.pml[L.n(q)];
+    // w = P (Rbar + c^2 Q) ; Rbar <- P w
+    const T wx = s * (rt[q].x + cc[q].x * v[q].x), wy = s * (rt[q].y + cc[q].y * v[q].y);
+    rt[q] = Cx<T>{wx, wy};
+    v[q] = Cx<T>{s * wx, s * wy};
+  }
+  L.store(a.Rb, v);
+  // D+ (dt rho0 w)
+  if (a.rho0) {
+    Cx<T> r0[E];
+    L.load_map(a.rho0, r0, false, (T)0, (T)1);
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = Cx<T>{a.dt * r0[q].x * rt[q].x, a.dt * r0[q].y * rt[q].y};
+  } else {
+    const T f = a.dt * a.rho0_s;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cscale(rt[q], f);
+  }
+  line_fft<T, N, false>(v, lds, tw, L.t);
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dp[L.n(q)]);
+  line_fft<T, N, true>(v, lds, tw, L.t);
+  // Ut = Ubar + D+(...) ; y = Psg Ut ; Ubar <- Psg y ; z = dt/rho0sg y
+  {
+    Cx<T> ub[E];
+    L.load(a.Ub, ub);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const T s = a.pml_sg[L.n(q)];
+      v[q] = Cx<T>{s * (ub[q].x + v[q].x), s * (ub[q].y + v[q].y)};
+      ub[q] = cscale(v[q], s);
+    }
+    L.store(a.Ub, ub);
+  }
+  if (a.rho0) {
+    Cx<T> r0[E], r1[E];
+    L.load_map(a.rho0, r0, false, (T)0, (T)1);
+    L.load_map_next(a.rho0, r1, false, (T)0, (T)1);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      v[q].x *= fdiv((T)2 * a.dt, r0[q].x + r1[q].x);
+      v[q].y *= fdiv((T)2 * a.dt, r0[q].y + r1[q].y);
+    }
+  } else {
+    const T f = a.dt / a.rho0_s;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cscale(v[q], f);
+  }
+  line_fft<T, N, false>(v, lds, tw, L.t);
+#pragma unroll
+  for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dm[L.n(q)]);
+  line_fft<T, N, true>(v, lds, tw, L.t);
+  if (a.accumulate) {
+    Cx<T> pb[E];
+    L.load(a.Pb, pb);
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cadd(v[q], pb[q]);
+  }
+  L.store(a.Pb, v);
+}
+
+// Real lines (pairs) of f -> half spectrum along x (the r2c stage alone).
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_r2c_x(const T* f, Cx<T>* S, int n_lines, int nhp, const Cx<T>* twt) {
+  constexpr int E = Plan<N>::E;
+  constexpr int P = LdsLayout<N>::PITCH;
+  constexpr int NH = N / 2 + 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
+  PassArgs<T> geo;
+  geo.n_lines = n_lines;
+  int lpw;
+  const LineCtx<T, N> L = make_ctx<T, N, true>(geo, blockIdx.x, lpw);
+  Cx<T>* lds = lds_all + L.c * P;
+  const Twiddles<T, N> tw = load_twiddles<T, N>(twt, L.t);
+  Cx<T> z[E];
+  L.load(f, z);
+  line_fft<T, N, false>(z, lds, tw, L.t);
+#pragma unroll
+  for (int q = 0; q < E; ++q) lds[LdsLayout<N>::at(L.n(q))] = z[q];
+  __syncthreads();
+  Cx<T>* __restrict__ Sw = S + 2LL * blockIdx.x * lpw * nhp;
+  const unsigned oa = (unsigned)(2 * L.c * nhp), ob = oa + nhp;
+#pragma unroll
+  for (int q = 0; q < E; ++q) {
+    const int k = L.n(q);
+    if (k < NH) {
+      const int km = (k == 0) ? 0 : N - k;
+      const Cx<T> zk = z[q];
+      const Cx<T> zm = lds[LdsLayout<N>::at(km)];
+      if (L.va) at32(Sw, oa + k) = Cx<T>{(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
+      if (L.vb) at32(Sw, ob + k) = Cx<T>{(zk.y + zm.y) * (T)0.5, (zm.x - zk.x) * (T)0.5};
+    }
+  }
+}
+
+// g += 2 c rho_sum Q (double) ; gpart[block] += sum over the block's elements of
+// c^2 rho_sum Q'.  Each block owns its slot and the grid is the same on every
+// call, so the sums are deterministic (no atomics).
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_grad(const T* c0sq, const T* rs, const T* Q, const T* Qd, double* g,
+                                                  double* gpart, long long n) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double c2 = (double)c0sq[i], r = (double)rs[i];
+    g[i] += 2.0 * sqrt(c2) * r * (double)Q[i];
+    acc += c2 * r * (double)Qd[i];
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) gpart[blockIdx.x] += red[0];
+}
+
+// source gradient: g[idx_j] += fac_j s_n[j] sum_d (Rbar_d + c^2 Q)[idx_j], fac_j = -2/(ndim c_j^3)
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_src(const T* r0, const T* r1, const T* r2, const T* c0sq, const T* Q,
+                                                 const long long* idx, const double* fac, const double* sig_col,
+                                                 double* g, int ndim, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const long long i = idx[j];
+  const double cq = (double)c0sq[i] * (double)Q[i];
+  double tot = (double)r0[i] + cq;
+  if (ndim > 1) tot += (double)r1[i] + cq;
+  if (ndim > 2) tot += (double)r2[i] + cq;
+  g[i] += fac[j] * sig_col[j] * tot;
+}
+
+// residuals r[t][j] = p_rec[t][j] - obs[t][j] and lpart[block] = sum of r^2/2 over the block's elements
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_resid(const T* prec, const double* obs, double* res, double* lpart,
+                                                   long long n) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double r = (double)prec[i] - obs[i];
+    res[i] = r;
+    acc += 0.5 * r * r;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
+}
+
+// Pbar[idx_j] += r_j  (sensor points are distinct)
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_scatter(T* Pb, const long long* idx, const double* r, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) Pb[idx[j]] += (T)r[j];
+}
+
+// g += gterm (= g_ref / N) and, for p0 runs, g += -2 p0 / (ndim c^3) sum_d Rbar_d
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_final(double* g, double gterm, const T* p0, const T* c0sq, const T* r0,
+                                                   const T* r1, const T* r2, int ndim, long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double v = g[i] + gterm;
+  if (p0) {
+    double s = (double)r0[i];
+    if (ndim > 1) s += (double)r1[i];
+    if (ndim > 2) s += (double)r2[i];
+    const double c = sqrt((double)c0sq[i]);
+    v += -2.0 * (double)p0[i] / (ndim * c * c * c) * s;
+  }
+  g[i] = v;
 }
 
 }  // namespace kw

@@ -13,6 +13,11 @@
 namespace kw {
 
 bool size_supported(long long n);
+// grid of the adjoint's reduction kernels (one partial-sum slot per block)
+inline unsigned adj_reduce_blocks(long long n) {
+  const long long nb = (n + 255) / 256;
+  return nb < 1 ? 1u : (nb < 1024 ? (unsigned)nb : 1024u);
+}
 int plan_tpl(int n);
 int plan_pitch(int n);
 
@@ -52,5 +57,14 @@ template <typename T>
 hipError_t launch_record_batch(hipStream_t s, const RecArgs<T>* args, const int* sens_sim, const int* sens_first,
                                int n_total, const int* t_dev);
 hipError_t launch_tick(hipStream_t s, int* t_dev);
+
+// Adjoint (gradient w.r.t. c0).
+template <typename T>
+hipError_t launch_adj_pass(int n, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const AdjArgs<T>& a);
+template <typename T>
+hipError_t launch_r2c(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const T* f, Cx<T>* S, int n_lines, int nhp,
+                      const Cx<T>* tw);
+template <typename T>
+hipError_t launch_adj_small(int which, hipStream_t s, const void* const* p, const double* dargs, long long n, int ndim);
 
 }  // namespace kw

@@ -21,6 +21,9 @@ static hipError_t spec_mode(int mode, dim3 g, dim3 b, size_t sh, hipStream_t s, 
     case 3:
       hipLaunchKernelGGL((k_spec_pass<T, N, 3>), g, b, sh, s, a);
       break;
+    case 5:
+      hipLaunchKernelGGL((k_spec_pass<T, N, 5>), g, b, sh, s, a);
+      break;
     default:
       return hipErrorInvalidValue;
   }

@@ -13,6 +13,8 @@ from .model import (AbstractKWaveGrid, Additive, AdditiveNoCorrection, Dirichlet
 from .builders import make_ball, make_bowl, make_circle, make_disc, tone_burst
 from .solver import Simulation, build_desc, kspace_first_order, quantize_map
 from .batch import Batch, kspace_first_order_batch
+from .adjoint import gradient_c0, misfit_rrule
+from . import tuning
 from ._lib import KWaveHIPError, LIB_PATH
 
 __all__ = [
@@ -21,5 +23,5 @@ __all__ = [
     "Additive", "AdditiveNoCorrection", "VALID_RECORD_FIELDS", "validate_record_fields",
     "is_homogeneous", "is_lossless", "is_nonlinear", "is_time_reversal", "has_p0",
     "has_pressure_source", "has_velocity_source", "kspace_first_order", "Simulation", "build_desc", "quantize_map",
-    "Batch", "kspace_first_order_batch", "tone_burst", "make_disc", "make_circle", "make_ball", "make_bowl", "KWaveHIPError", "LIB_PATH",
+    "Batch", "kspace_first_order_batch", "gradient_c0", "misfit_rrule", "tuning", "tone_burst", "make_disc", "make_circle", "make_ball", "make_bowl", "KWaveHIPError", "LIB_PATH",
 ]

@@ -10,7 +10,7 @@ from typing import Tuple
 
 import numpy as np
 
-from .builders import make_circle, make_disc, tone_burst
+from .builders import make_bowl, make_circle, make_disc, tone_burst
 from .model import KWaveGrid, KWaveMedium, KWaveSensor, KWaveSource, make_time
 
 
@@ -150,3 +150,91 @@ def config3(n_sims: int = 64, Nt: int = 1500, seed: int = 42, sizes=(256, 384, 5
         sen = KWaveSensor(mask=sm, record=["p"])
         sims.append((kg, med, src, sen))
     return sims, dict(pml_size=20, pml_alpha=2.0, smooth_p0=False)
+
+
+def config4(N: int = 512, Nt: int = 2000, pml: int = 24, amplitude: float = 1e6, seed: int = 42,
+            n_points: int = 4096):
+    """3-D layered FP16-map workload (N^3, dx = 0.2 mm, PML 24).
+
+    Six layers split at 5 seeded z-planes (uniform in [pml+1, N-pml], sorted),
+    c0 ~ U[1400,1700] and rho0 ~ U[950,1150] per layer, plus N(0, 10 m/s)
+    noise on c0 (numpy default_rng(seed)); B/A = 6, alpha0 = 0.6, y = 1.5,
+    :stokes with scalar alpha (an array alpha fails in the reference,
+    first_order.jl:588-592).  Bowl source make_bowl(N,N,N,(N/2,N/2,175),150,120,
+    (N/2,N/2,0)) (1-based), a 1 MHz 10-cycle tone burst of `amplitude` Pa.
+    Two sensor sets (one KWaveSensor cannot hold both, sensor.jl:31-38): the
+    whole volume recording p_max, and `n_points` seeded points recording p.
+    Returns (kgrid, medium, source, [sensor_pmax, sensor_points], kwargs).
+    """
+    rng = SplitMix64(seed)
+    kg = KWaveGrid(N, 2e-4, N, 2e-4, N, 2e-4)
+    cuts = sorted(rng.integer(pml + 1, N - pml) for _ in range(5))
+    c_lay = [rng.uniform(1400.0, 1700.0) for _ in range(6)]
+    r_lay = [rng.uniform(950.0, 1150.0) for _ in range(6)]
+    z = np.arange(1, N + 1)
+    layer = np.searchsorted(np.asarray(cuts), z, side="right")          # 0..5 along z
+    c0 = np.broadcast_to(np.asarray(c_lay)[layer][None, None, :], (N, N, N)).astype(np.float64)
+    noise = np.random.default_rng(seed).standard_normal((N, N, N), dtype=np.float32)
+    c0 = c0 + 10.0 * noise
+    del noise
+    rho0 = np.broadcast_to(np.asarray(r_lay)[layer][None, None, :], (N, N, N)).astype(np.float64)
+    make_time(kg, c0)
+    kg.Nt = Nt
+    kg.t_array = np.arange(Nt) * kg.dt
+    med = KWaveMedium(sound_speed=c0, density=rho0, alpha_coeff=0.6, alpha_power=1.5, alpha_mode="stokes",
+                      BonA=6.0)
+    h = N // 2
+    pm = make_bowl(N, N, N, (h, h, 175), 150, 120, (h, h, 0))
+    burst = amplitude * tone_burst(1 / kg.dt, 1e6, 10)
+    n_src = int(pm.sum())
+    sig = np.zeros((n_src, burst.size + 1))
+    sig[:, : burst.size] = burst[None, :]
+    src = KWaveSource(p_mask=pm, p=sig)
+    s_all = KWaveSensor(mask=np.ones((N, N, N), dtype=bool), record=["p_max"])
+    pts = np.zeros(N ** 3, dtype=bool)
+    pts[np.random.default_rng(seed + 1).choice(N ** 3, size=n_points, replace=False)] = True
+    s_pts = KWaveSensor(mask=pts.reshape((N, N, N), order="F"), record=["p"])
+    return kg, med, src, [s_all, s_pts], dict(pml_size=pml, pml_alpha=2.0, smooth_p0=False,
+                                              medium_storage=np.float16)
+
+
+def config5(N: int = 512, Nt: int = 1600, n_elements: int = 128, radius: float = 220.0, seed: int = 42):
+    """Adjoint workload: 2-D N x N, dx = 0.1 mm, rho0 = 1000, lossless linear.
+
+    True model: c0 = 1500 plus 6 seeded discs (radius integer in [10,40],
+    c0 ~ U[1450,1600], centres uniform in the ring's interior); starting
+    model: homogeneous 1500 given as a map.  dt comes from the true model
+    (CFL 0.3 on its maximum) and is held fixed for every model.  Ring of
+    `n_elements` points at round(N/2 + radius (cos, sin)(2 pi k / n)) (1-based);
+    firing k drives element k with a 1.5 MHz 2-cycle tone burst; every
+    element records p.  Returns (kgrid, c_true, c_start, firings, kwargs) where
+    firings is a list of (source, sensor).
+    """
+    rng = SplitMix64(seed)
+    kg = KWaveGrid(N, 1e-4, N, 1e-4)
+    c_true = np.full((N, N), 1500.0)
+    h = N / 2
+    for _ in range(6):
+        r = rng.integer(10, 40)
+        cx = rng.uniform(h - radius / 2, h + radius / 2)
+        cy = rng.uniform(h - radius / 2, h + radius / 2)
+        cv = rng.uniform(1450.0, 1600.0)
+        c_true[make_disc(N, N, int(round(cx)), int(round(cy)), r)] = cv
+    make_time(kg, c_true)
+    kg.Nt = Nt
+    kg.t_array = np.arange(Nt) * kg.dt
+    ang = 2 * np.pi * np.arange(n_elements) / n_elements
+    ex = np.round(h + radius * np.cos(ang)).astype(int) - 1          # 0-based
+    ey = np.round(h + radius * np.sin(ang)).astype(int) - 1
+    ring = np.zeros((N, N), dtype=bool)
+    ring[ex, ey] = True
+    burst = tone_burst(1 / kg.dt, 1.5e6, 2)
+    firings = []
+    for k in range(n_elements):
+        pm = np.zeros((N, N), dtype=bool)
+        pm[ex[k], ey[k]] = True
+        sig = np.zeros((1, burst.size + 1))
+        sig[0, : burst.size] = burst
+        firings.append((KWaveSource(p_mask=pm, p=sig), KWaveSensor(mask=ring, record=["p"])))
+    c_start = np.full((N, N), 1500.0)
+    return kg, c_true, c_start, firings, dict(pml_size=20, pml_alpha=2.0, smooth_p0=False)

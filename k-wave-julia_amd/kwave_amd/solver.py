@@ -114,6 +114,16 @@ class Simulation:
         _lib.check(self._lib.kw_sim_profile(self._h, int(n_steps), m, C.byref(nc), names, ms, cnt))
         return {names[i].decode(): {"ms": ms[i], "launches": int(cnt[i])} for i in range(nc.value)}
 
+    def grad_c0(self, p_obs: np.ndarray, checkpoint_every: int = 50):
+        """(loss, dL/dc0) of L = 1/2 sum (p - p_obs)^2 by the checkpointed adjoint."""
+        shape = tuple(self.desc.N[a] for a in range(self.desc.ndim))
+        obs = np.asfortranarray(np.asarray(p_obs, dtype=np.float64))
+        g = np.empty(int(np.prod(shape)), dtype=np.float64)
+        loss = C.c_double()
+        _lib.check(self._lib.kw_sim_grad_c0(self._h, obs.ravel(order="F").ctypes.data_as(_DP), int(checkpoint_every),
+                                            g.ctypes.data_as(_DP), C.byref(loss)))
+        return loss.value, g.reshape(shape, order="F")
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             _lib.check(self._lib.kw_sim_destroy(self._h))

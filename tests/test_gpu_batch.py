@@ -48,3 +48,42 @@ def test_batch_3d():
         for k in outs[i].keys():
             e = rel_l2(outs[i][k], ref[k])
             assert e <= 1e-10, (i, k, e)
+
+
+def _table_for(lengths, lpw):
+    return "".join(f"{p} {c} {n} {lpw}\n" for p in (4, 8) for c in range(6) for n in lengths)
+
+
+def test_batch_with_tuned_shapes():
+    """Non-default lines per workgroup (through the tuning table) keep parity."""
+    print("\n[case] batch 2d f64 with 2 lines per workgroup", flush=True)
+    from kwave_amd import tuning
+    sims, kw = _sims([("hetc_absorb_psrc", (64, 64)), ("stokes_p0_psrc", (64, 96)), ("psrc", (128, 48))], nt=30)
+    tuning.clear_table()
+    try:
+        from kwave_amd import _lib
+        _lib.check(_lib.load().kw_tune_import(_table_for((48, 64, 96, 128), 2).encode()))
+        outs = K.kspace_first_order_batch([tuple(copy.deepcopy(s)) for s in sims], data_cast=np.float64, **kw)
+        single = K.kspace_first_order(*copy.deepcopy(sims[0]), data_cast=np.float64, **kw)
+    finally:
+        tuning.clear_table()
+    for i, (kg, med, src, sen) in enumerate(sims):
+        ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
+        for k in outs[i].keys():
+            assert rel_l2(outs[i][k], ref[k]) <= 1e-10, (i, k)
+    assert rel_l2(single["p"], O.kspace_first_order(copy.deepcopy(sims[0][0]), *sims[0][1:], **kw)["p"]) <= 1e-10
+
+
+def test_autotune_fills_table():
+    print("\n[case] autotune 2d 64x96 f32", flush=True)
+    from kwave_amd import tuning
+    tuning.clear_table()
+    try:
+        sims, kw = _sims([("hetc_absorb_psrc", (64, 96))], nt=20)
+        kg, med, src, sen = sims[0]
+        t = tuning.autotune(kg, med, src, sen, steps=3, data_cast=np.float32, **kw)
+    finally:
+        tuning.clear_table()
+    print(f"    table {t}", flush=True)
+    for key in [(4, "dir_x", 64), (4, "dir_y", 96), (4, "spec_y", 96), (4, "c2r_x", 64)]:
+        assert key in t and t[key] >= 1

@@ -113,3 +113,23 @@ def test_simulation_output_mirror():
     out = K.SimulationOutput({"p": np.zeros((2, 3)), "p_max": np.zeros(2)})
     assert out[":p"].shape == (2, 3) and "p_max" in out and out.haskey("p") and set(out.keys()) == {"p", "p_max"}
     assert repr(out) == "SimulationOutput with fields: p, p_max"
+
+
+def test_tuning_table_text_round_trip():
+    """kw_tune_import / kw_tune_export / kw_tune_clear need no device."""
+    from kwave_amd import _lib, tuning
+    lib = _lib.load()
+    tuning.clear_table()
+    try:
+        _lib.check(lib.kw_tune_import(b"4 0 256 8\n8 3 96 2\n"))
+        assert tuning.table() == {(4, "dir_x", 256): 8, (8, "spec_y", 96): 2}
+        with pytest.raises(_lib.KWaveHIPError):
+            _lib.check(lib.kw_tune_import(b"4 0 257 8\n"))       # no plan for 257
+        with pytest.raises(_lib.KWaveHIPError):
+            _lib.check(lib.kw_tune_import(b"4 9 256 8\n"))       # no such pass
+        with pytest.raises(_lib.KWaveHIPError):
+            _lib.check(lib.kw_tune_import(b"4 0 256 x\n"))
+        assert len(tuning.table()) == 2                          # failed imports add nothing
+    finally:
+        tuning.clear_table()
+    assert tuning.table() == {}
