@@ -128,13 +128,9 @@ def bench_batch(a):
 
     K.tuning.clear_table()
     default_secs = run_batch()
-    # autotuner: one simulation of each distinct grid size (untimed)
+    # autotuner: trials of the whole batch (untimed)
     tt = time.perf_counter()
-    seen = set()
-    for s in sims:
-        if s[0].Nx not in seen:
-            seen.add(s[0].Nx)
-            K.tuning.autotune(*s, steps=10, data_cast=np.float32, **kw)
+    K.tuning.autotune_batch(sims, steps=10, data_cast=np.float32, **kw)
     tune_s = time.perf_counter() - tt
     table = K.tuning.table()
     t1 = 0.0

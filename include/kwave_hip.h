@@ -189,15 +189,18 @@ int32_t kw_sim_profile(kw_sim* sim, int64_t n_steps, int32_t max_classes, int32_
 /*
  * Launch-shape autotuner (no counterpart in the reference, whose FFTW plans
  * are tuned by FFTW.MEASURE in create_fft_plans, fft_utils.jl:44-62).  Runs
- * timed trials of `desc` (steps timed steps each; time reversal ignored) with
- * 1, 1/2, 1/4 and 1/8 of the default lines per workgroup for every pass and
- * keeps the fastest per (precision, pass, line length) in a process-wide
- * table that later kw_sim_create / kw_batch_create calls read.
+ * timed trials (steps timed steps each; time reversal ignored) with 1, 1/2,
+ * 1/4 and 1/8 of the default lines per workgroup for every pass and keeps
+ * the fastest per (precision, pass, line length) in a process-wide table
+ * that later kw_sim_create / kw_batch_create calls read.  With n_sims == 1
+ * the trials run descs[0] as a single simulation; with n_sims > 1 they run
+ * the whole array as one batch (kw_batch_create), so the table fits that
+ * batch.
  * kw_tune_export writes the table as text lines "precision pass n lines"
  * (*len = its length; buf gets at most cap-1 bytes plus a NUL);
  * kw_tune_import merges such text; kw_tune_clear empties the table.
  */
-int32_t kw_autotune(kw_ctx* ctx, const kw_sim_desc* desc, int32_t steps);
+int32_t kw_autotune(kw_ctx* ctx, const kw_sim_desc* descs, int32_t n_sims, int32_t steps);
 int32_t kw_tune_export(char* buf, int64_t cap, int64_t* len);
 int32_t kw_tune_import(const char* text);
 int32_t kw_tune_clear(void);

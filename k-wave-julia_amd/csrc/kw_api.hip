@@ -1458,11 +1458,50 @@ class Batch final : public BatchBase {
   int size() const override { return (int)sims_.size(); }
   long long steps_done() const override { return t_; }
 
+  // Autotuner trial in batch form: `warm` untimed steps, then the mean
+  // device time per step of every group over `steps` steps, reported per
+  // (pass class, line length) with the lines per workgroup it ran with.
+  void trial(long long warm, long long steps, std::map<std::array<int, 2>, std::pair<double, int>>& out) {
+    HIPC(hipSetDevice(dev_));
+    for (long long i = 0; i < warm && t_ < Nt_; ++i, ++t_) enqueue_step();
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    long long done = 0;
+    for (; done < steps && t_ < Nt_; ++done, ++t_) enqueue_step(&ev);
+    HIPC(hipStreamSynchronize(stream_));
+    const size_t ng = groups_.size();
+    std::vector<double> tot(ng, 0.0);
+    for (size_t k = 0; k < ev.size(); ++k) {
+      float e = 0.f;
+      HIPC(hipEventElapsedTime(&e, ev[k].first, ev[k].second));
+      tot[k % ng] += e;
+      (void)hipEventDestroy(ev[k].first);
+      (void)hipEventDestroy(ev[k].second);
+    }
+    for (size_t k = 0; k < ng; ++k) {
+      const Group& g = *groups_[k];
+      const int cls = tune_class(g.kind);
+      auto& slot = out[{cls, g.n}];
+      slot.first += tot[k] / (double)std::max<long long>(done, 1);
+      slot.second = g.req;
+    }
+  }
+
  private:
+  static int tune_class(int k) {
+    switch (k) {
+      case DIR_Y: return TC_DIR_Y;
+      case DIR_Z: return TC_DIR_Z;
+      case DIR_X: return TC_DIR_X;
+      case SPEC_Z: return TC_SPEC_Z;
+      case C2R: return TC_C2R;
+      default: return TC_SPEC_Y;
+    }
+  }
   enum Kind { DIR_Y, DIR_Z, DIR_X, SPEC_Y_FWD, SPEC_Z, SPEC_Y_INV, SPEC_Y, C2R };
   struct Group {
     Kind kind;
     int n;
+    int req = 1;   // lines per workgroup asked for (tuning key)
     dim3 block;
     size_t shmem;
     unsigned nwg = 0;
@@ -1501,6 +1540,7 @@ class Batch final : public BatchBase {
             raise(KW_ERR_STATE, "batched simulations disagree on the workgroup shape of a line length");
           g.block = c.block;
           g.shmem = c.shmem;
+          g.req = c.req;
           wg_first[i] = (int)g.nwg;
           for (unsigned w = 0; w < c.grid.x; ++w) wg_sim.push_back((int)i);
           g.nwg += c.grid.x;
@@ -1581,13 +1621,19 @@ class Batch final : public BatchBase {
     upload(sens_first_, sens_first);
   }
 
-  void enqueue_step() {
+  void enqueue_step(std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev = nullptr) {
     HIPC(launch_tick(stream_, reinterpret_cast<int*>(t_dev_.p)));
     for (auto& gp : groups_) {
       Group& g = *gp;
+      if (ev) {
+        ev->emplace_back(nullptr, nullptr);
+        HIPC(hipEventCreate(&ev->back().first));
+        HIPC(hipEventCreate(&ev->back().second));
+        HIPC(hipEventRecord(ev->back().first, stream_));
+      }
       switch (g.kind) {
         case DIR_Y: case DIR_Z: case DIR_X:
-          HIPC(launch_dir_batch<T>(g.n, g.kind == DIR_X, g.nwg, g.block, g.shmem, stream_, reinterpret_cast<PassArgs<T>*>(g.args.p),
+          HIPC(launch_dir_batch<T>(g.n, g.kind == DIR_X, g.kind == DIR_X && g.two, g.nwg, g.block, g.shmem, stream_, reinterpret_cast<PassArgs<T>*>(g.args.p),
                                    reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p), reinterpret_cast<int*>(t_dev_.p)));
           break;
         case C2R:
@@ -1600,6 +1646,7 @@ class Batch final : public BatchBase {
                                     reinterpret_cast<SpecArgs<T>*>(g.args.p), reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p)));
         }
       }
+      if (ev) HIPC(hipEventRecord(ev->back().second, stream_));
     }
     if (n_sens_ > 0)
       HIPC(launch_record_batch<T>(stream_, reinterpret_cast<RecArgs<T>*>(rec_args_.p), reinterpret_cast<int*>(sens_sim_.p), reinterpret_cast<int*>(sens_first_.p),
@@ -1675,6 +1722,44 @@ void autotune(int dev, const kw_sim_desc& d0, int steps) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   for (int c = 0; c < TC_COUNT; ++c)
     if (best_n[c] > 0) g_tune[{(int)sizeof(T), c, best_n[c]}] = best_lp[c];
+}
+
+// Batch form of the autotuner: the whole batch is built and timed for each
+// choice, so the table holds the shapes that are fastest when many
+// simulations share each launch.
+template <typename T>
+void autotune_batch(int dev, const kw_sim_desc* d0, int n, int steps) {
+  std::vector<kw_sim_desc> d(d0, d0 + n);
+  const long long warm = 2;
+  for (auto& x : d) {
+    x.tr_data = nullptr;
+    x.Nt = warm + steps;
+  }
+  std::map<std::array<int, 2>, std::pair<double, int>> best;
+  std::vector<int> prev;
+  for (int div = 1; div <= 8; div *= 2) {
+    std::unique_ptr<Batch<T>> b;
+    g_tune_div = div;
+    try {
+      b.reset(new Batch<T>(dev, d.data(), n, false));
+    } catch (...) {
+      g_tune_div = 0;
+      throw;
+    }
+    g_tune_div = 0;
+    std::map<std::array<int, 2>, std::pair<double, int>> r;
+    b->trial(warm, steps, r);
+    std::vector<int> lp;
+    for (auto& kv : r) {
+      lp.push_back(kv.second.second);
+      auto it = best.find(kv.first);
+      if (it == best.end() || kv.second.first < it->second.first) best[kv.first] = kv.second;
+    }
+    if (lp == prev) break;  // no class changed its shape any more
+    prev = lp;
+  }
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  for (auto& kv : best) g_tune[{(int)sizeof(T), kv.first[0], kv.first[1]}] = kv.second.second;
 }
 
 }  // namespace
@@ -1820,17 +1905,23 @@ int32_t kw_sim_profile(kw_sim* sim, int64_t n_steps, int32_t max_classes, int32_
   KW_GUARD_END
 }
 
-int32_t kw_autotune(kw_ctx* ctx, const kw_sim_desc* desc, int32_t steps) {
+int32_t kw_autotune(kw_ctx* ctx, const kw_sim_desc* descs, int32_t n_sims, int32_t steps) {
   KW_GUARD_BEGIN
-  if (!ctx || !desc) raise(KW_ERR_INVALID, "null argument");
+  if (!ctx || !descs) raise(KW_ERR_INVALID, "null argument");
+  if (n_sims < 1) raise(KW_ERR_INVALID, "n_sims must be >= 1");
   if (steps < 1) steps = 1;
   HIPC(hipSetDevice(ctx->device));
-  if (desc->precision == 4)
-    autotune<float>(ctx->device, *desc, steps);
-  else if (desc->precision == 8)
-    autotune<double>(ctx->device, *desc, steps);
-  else
-    raise(KW_ERR_INVALID, "precision must be 4 or 8");
+  const int prec = descs[0].precision;
+  if (prec != 4 && prec != 8) raise(KW_ERR_INVALID, "precision must be 4 or 8");
+  for (int i = 1; i < n_sims; ++i)
+    if (descs[i].precision != prec) raise(KW_ERR_INVALID, "a batch needs one precision");
+  if (n_sims == 1) {
+    if (prec == 4) autotune<float>(ctx->device, descs[0], steps);
+    else autotune<double>(ctx->device, descs[0], steps);
+  } else {
+    if (prec == 4) autotune_batch<float>(ctx->device, descs, n_sims, steps);
+    else autotune_batch<double>(ctx->device, descs, n_sims, steps);
+  }
   return KW_OK;
   KW_GUARD_END
 }

@@ -256,13 +256,13 @@ KW_PLAN(160, 8, 20, 4, 2)
 KW_PLAN(192, 8, 24, 8)
 KW_PLAN(256, 16, 16, 16)
 KW_PLAN(320, 16, 20, 4, 4)
-KW_PLAN(384, 16, 24, 8, 2)
-KW_PLAN(512, 16, 32, 16)
+KW_PLAN(384, 32, 12, 4, 4, 2)
+KW_PLAN(512, 64, 8, 8, 8)
 KW_PLAN(640, 32, 20, 4, 4, 2)
-KW_PLAN(768, 32, 24, 8, 4)
-KW_PLAN(1024, 32, 32, 32)
-KW_PLAN(2048, 64, 32, 8, 8)
-KW_PLAN(4096, 128, 32, 16, 8)
+KW_PLAN(768, 64, 12, 4, 4, 4)
+KW_PLAN(1024, 64, 16, 16, 4)
+KW_PLAN(2048, 128, 16, 16, 8)
+KW_PLAN(4096, 256, 16, 16, 16)
 #undef KW_PLAN
 
 // LDS slice of one line: one complex of padding per 16 keeps the

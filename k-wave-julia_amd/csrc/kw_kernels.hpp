@@ -19,6 +19,8 @@
 #pragma once
 #include <hip/hip_fp16.h>
 
+#include <type_traits>
+
 #include "kw_fft.hpp"
 
 namespace kw {
@@ -120,8 +122,10 @@ template <typename T, int N>
 struct LineCtx {
   static constexpr int TPL = Plan<N>::TPL;
   static constexpr int E = Plan<N>::E;
+  using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   int t, c;
   bool va, vb;          // real line a / b valid
+  bool vec = false;     // strided pairs are adjacent and aligned: one 2-element access per pair
   long long wgb;        // workgroup base element offset (uniform)
   unsigned oa, ob;      // lane offsets of lines a / b from wgb
   unsigned sa;          // element stride along the line
@@ -137,6 +141,15 @@ struct LineCtx {
   // `fill` is what an absent line of the pair holds (1 for divisors like rho0)
   __device__ __forceinline__ void load(const T* __restrict__ f0, Cx<T> (&v)[E], T fill = (T)0) const {
     const T* __restrict__ f = f0 + wgb;
+    if (vec) {  // va == vb here
+      const V2* __restrict__ f2 = reinterpret_cast<const V2*>(f);
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const V2 r = va ? at32(f2, ia(q) >> 1) : V2{fill, fill};
+        v[q] = Cx<T>{r.x, r.y};
+      }
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       v[q].x = va ? at32(f, ia(q)) : fill;
@@ -145,11 +158,27 @@ struct LineCtx {
   }
   __device__ __forceinline__ void store(T* __restrict__ f0, const Cx<T> (&v)[E]) const {
     T* __restrict__ f = f0 + wgb;
+    if (vec) {
+      V2* __restrict__ f2 = reinterpret_cast<V2*>(f);
+#pragma unroll
+      for (int q = 0; q < E; ++q)
+        if (va) at32(f2, ia(q) >> 1) = V2{v[q].x, v[q].y};
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       if (va) at32(f, ia(q)) = v[q].x;
       if (vb) at32(f, ib(q)) = v[q].y;
     }
+  }
+  // pair of map values at element offset i (even) of the workgroup base
+  __device__ __forceinline__ Cx<T> map_pair(const void* f, unsigned i, bool f16, T off) const {
+    if (f16) {
+      const __half2 h = at32(reinterpret_cast<const __half2*>(reinterpret_cast<const __half*>(f) + wgb), i >> 1);
+      return Cx<T>{off + (T)__low2float(h), off + (T)__high2float(h)};
+    }
+    const V2 r = at32(reinterpret_cast<const V2*>(reinterpret_cast<const T*>(f) + wgb), i >> 1);
+    return Cx<T>{r.x, r.y};
   }
   // static medium map in T, or FP16 offset from `off` (converted to T)
   __device__ __forceinline__ T map_at(const void* f, unsigned i, bool f16, T off) const {
@@ -157,6 +186,11 @@ struct LineCtx {
                : at32(reinterpret_cast<const T*>(f) + wgb, i);
   }
   __device__ __forceinline__ void load_map(const void* f, Cx<T> (&v)[E], bool f16, T off, T fill) const {
+    if (vec) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = va ? map_pair(f, ia(q), f16, off) : Cx<T>{fill, fill};
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       v[q].x = va ? map_at(f, ia(q), f16, off) : fill;
@@ -164,6 +198,11 @@ struct LineCtx {
     }
   }
   __device__ __forceinline__ void load_map_next(const void* f, Cx<T> (&v)[E], bool f16, T off, T fill) const {
+    if (vec) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = va ? map_pair(f, oa + nx(q), f16, off) : Cx<T>{fill, fill};
+      return;
+    }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       v[q].x = va ? map_at(f, oa + nx(q), f16, off) : fill;
@@ -202,6 +241,8 @@ __device__ __forceinline__ LineCtx<T, N> make_ctx(const PassArgs<T>& a, int wg, 
     L.oa = (unsigned)(2 * L.c);
     L.ob = L.oa + 1;
     L.sa = (unsigned)a.Sa;
+    // even Nx, Sa and So: every valid pair is complete and 2-element aligned
+    L.vec = ((a.Nx | a.Sa | a.So) & 1) == 0;
   }
   return L;
 }
@@ -239,7 +280,10 @@ __device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)
 // ---------------------------------------------------------------------------
 // Directional pass (contiguous x lines or strided y/z lines)
 // ---------------------------------------------------------------------------
-template <typename T, int N, bool CONTIG>
+// TWO (x pass only): the variant that can also form the second spectrum of
+// the homogeneous :stokes branch; without it the kernel carries no registers
+// for that path.
+template <typename T, int N, bool CONTIG, bool TWO = false>
 __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int t_index) {
   constexpr int E = Plan<N>::E;
   constexpr int P = LdsLayout<N>::PITCH;
@@ -363,8 +407,21 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
       for (int q = 0; q < E; ++q) f[q] = cadd(f[q], w[q]);
     }
     if (a.rho_sum_out) L.store(a.rho_sum_out, f);
-    Cx<T> g[E];  // second spectrum input (:stokes)
-    if (a.S2 != nullptr) {
+    Cx<T> g[TWO ? E : 1];  // second spectrum input (:stokes)
+    const bool two = TWO && a.S2 != nullptr;   // uniform per workgroup
+    auto eff = [&]() {      // f <- rho (1 + B/A rho / (2 rho0))
+      Cx<T> bb[E];
+      if (a.bona) L.load(a.bona, bb);
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const T ba = a.bona ? bb[q].x : a.bona_s, bbv = a.bona ? bb[q].y : a.bona_s;
+        const T ra = a.rho0 ? r0[q].x : a.rho0_s, rb = a.rho0 ? r0[q].y : a.rho0_s;
+        f[q].x = f[q].x * ((T)1 + ba / (T)2 * fdiv(f[q].x, ra));
+        f[q].y = f[q].y * ((T)1 + bbv / (T)2 * fdiv(f[q].y, rb));
+      }
+    };
+    if constexpr (TWO) {
+     if (two) {
       Cx<T> pr[E];
       L.load(a.rho_prev, pr);
       L.store(a.rho_prev, f);
@@ -392,16 +449,11 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
           g[q].y = (f[q].y - pr[q].y) * a.inv_dt;
         }
       }
+     } else if (a.nonlinear) {
+      eff();
+     }
     } else if (a.nonlinear) {
-      Cx<T> bb[E];
-      if (a.bona) L.load(a.bona, bb);
-#pragma unroll
-      for (int q = 0; q < E; ++q) {
-        const T ba = a.bona ? bb[q].x : a.bona_s, bbv = a.bona ? bb[q].y : a.bona_s;
-        const T ra = a.rho0 ? r0[q].x : a.rho0_s, rb = a.rho0 ? r0[q].y : a.rho0_s;
-        f[q].x = f[q].x * ((T)1 + ba / (T)2 * fdiv(f[q].x, ra));
-        f[q].y = f[q].y * ((T)1 + bbv / (T)2 * fdiv(f[q].y, rb));
-      }
+      eff();
     }
     if (a.c0sq) {
       Cx<T> cc[E];
@@ -423,12 +475,12 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
     // B_k = (Z_k - conj Z_{N-k})/(2i), k = 0..N/2.
     constexpr int NH = N / 2 + 1;
 #pragma unroll 1
-    for (int s = 0; s < 2; ++s) {
-      if (s == 1 && a.S2 == nullptr) break;
+    for (int s = 0; s < (TWO ? 2 : 1); ++s) {
+      if (s == 1 && !two) break;
       Cx<T>* Sdst = s == 0 ? a.S1 : a.S2;
       Cx<T> z[E];
 #pragma unroll
-      for (int q = 0; q < E; ++q) z[q] = s == 0 ? f[q] : g[q];
+      for (int q = 0; q < E; ++q) z[q] = s == 0 ? f[q] : g[TWO ? q : 0];
       line_fft<T, N, false>(z, lds, tw, L.t);
 #pragma unroll
       for (int q = 0; q < E; ++q) lds[LdsLayout<N>::at(L.n(q))] = z[q];
@@ -453,20 +505,20 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
   }
 }
 
-template <typename T, int N, bool CONTIG>
+template <typename T, int N, bool CONTIG, bool TWO = false>
 __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
-  dir_pass_body<T, N, CONTIG>(a, blockIdx.x, a.t);
+  dir_pass_body<T, N, CONTIG, TWO>(a, blockIdx.x, a.t);
 }
 
 // Batched form: one launch covers the workgroups of many simulations of the
 // same line length; wg_sim maps a workgroup to its simulation, wg_first is
 // each simulation's first workgroup, *t_dev the current 1-based step.
-template <typename T, int N, bool CONTIG>
+template <typename T, int N, bool CONTIG, bool TWO = false>
 __global__ void __launch_bounds__(256) k_dir_pass_batch(const PassArgs<T>* __restrict__ args,
                                                         const int* __restrict__ wg_sim,
                                                         const int* __restrict__ wg_first, const int* t_dev) {
   const int s = wg_sim[blockIdx.x];
-  dir_pass_body<T, N, CONTIG>(args[s], blockIdx.x - wg_first[s], *t_dev);
+  dir_pass_body<T, N, CONTIG, TWO>(args[s], blockIdx.x - wg_first[s], *t_dev);
 }
 
 // ---------------------------------------------------------------------------

@@ -45,7 +45,7 @@ hipError_t launch_filter(hipStream_t s, const T* p_ts, const FiltArgs& a);
 
 // Batched forms (one launch over the workgroups of many simulations).
 template <typename T>
-hipError_t launch_dir_batch(int n, bool contig, unsigned nwg, dim3 b, size_t sh, hipStream_t s, const PassArgs<T>* args,
+hipError_t launch_dir_batch(int n, bool contig, bool two, unsigned nwg, dim3 b, size_t sh, hipStream_t s, const PassArgs<T>* args,
                             const int* wg_sim, const int* wg_first, const int* t_dev);
 template <typename T>
 hipError_t launch_spec_batch(int n, int mode, bool two, unsigned nwg, dim3 b, size_t sh, hipStream_t s,

@@ -4,16 +4,21 @@
 namespace kw {
 
 template <>
-hipError_t launch_dir_batch<double>(int n, bool contig, unsigned nwg, dim3 b, size_t sh, hipStream_t s,
+hipError_t launch_dir_batch<double>(int n, bool contig, bool two, unsigned nwg, dim3 b, size_t sh, hipStream_t s,
                                  const PassArgs<double>* args, const int* wg_sim, const int* wg_first,
                                  const int* t_dev) {
   switch (n) {
-#define X(n_)                                                                                        \
-  case n_:                                                                                           \
-    if (contig)                                                                                      \
-      hipLaunchKernelGGL((k_dir_pass_batch<double, n_, true>), dim3(nwg), b, sh, s, args, wg_sim, wg_first, t_dev);  \
-    else                                                                                             \
-      hipLaunchKernelGGL((k_dir_pass_batch<double, n_, false>), dim3(nwg), b, sh, s, args, wg_sim, wg_first, t_dev); \
+#define X(n_)                                                                                              \
+  case n_:                                                                                                 \
+    if (contig && two)                                                                                     \
+      hipLaunchKernelGGL((k_dir_pass_batch<double, n_, true, true>), dim3(nwg), b, sh, s, args, wg_sim, wg_first, \
+                         t_dev);                                                                           \
+    else if (contig)                                                                                       \
+      hipLaunchKernelGGL((k_dir_pass_batch<double, n_, true, false>), dim3(nwg), b, sh, s, args, wg_sim,        \
+                         wg_first, t_dev);                                                                 \
+    else                                                                                                   \
+      hipLaunchKernelGGL((k_dir_pass_batch<double, n_, false>), dim3(nwg), b, sh, s, args, wg_sim, wg_first,    \
+                         t_dev);                                                                           \
     return hipGetLastError();
     KW_SIZES(X)
 #undef X

@@ -131,7 +131,7 @@ def load():
             "kw_sim_grad_c0": ([P, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_double),
                                 C.POINTER(C.c_double)], C.c_int32),
             "kw_sim_destroy": ([P], C.c_int32),
-            "kw_autotune": ([P, C.POINTER(SimDesc), C.c_int32], C.c_int32),
+            "kw_autotune": ([P, C.POINTER(SimDesc), C.c_int32, C.c_int32], C.c_int32),
             "kw_tune_export": ([C.c_char_p, C.c_int64, C.POINTER(C.c_int64)], C.c_int32),
             "kw_tune_import": ([C.c_char_p], C.c_int32),
             "kw_tune_clear": ([], C.c_int32),

@@ -78,23 +78,12 @@ def kspace_first_order_batch(sims: Sequence[Tuple], *, data_cast=np.float32, use
 
     Keyword arguments are those of ``kspace_first_order`` and apply to every
     simulation; all simulations must share dimensionality and Nt.  With
-    ``autotune`` the launch shapes are first timed on one simulation of each
-    distinct grid size (``tune_cache``: text file the table is kept in).
+    ``autotune`` the launch shapes are first timed on the whole batch
+    (``tune_cache``: text file the table is kept in).
     """
     if autotune:
-        from .tuning import autotune as _tune, load_table, save_table
-        import os
-        if tune_cache and os.path.exists(tune_cache):
-            load_table(tune_cache)
-        seen = set()
-        for kgrid, medium, source, sensor in sims:
-            shape = tuple(kgrid.grid_size())
-            if shape in seen:
-                continue
-            seen.add(shape)
-            _tune(kgrid, medium, source, sensor, data_cast=data_cast, device=device, **kwargs)
-        if tune_cache:
-            save_table(tune_cache)
+        from .tuning import autotune_batch
+        autotune_batch(sims, data_cast=data_cast, device=device, cache=tune_cache, **kwargs)
     descs, keep, infos = [], [], []
     for kgrid, medium, source, sensor in sims:
         d, k, info = build_desc(kgrid, medium, source, sensor, data_cast=data_cast, **kwargs)

@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
-from typing import Dict, Tuple
+from typing import Dict, Sequence, Tuple
 
 import numpy as np
 
@@ -33,7 +33,30 @@ def autotune(kgrid, medium, source, sensor, *, steps: int = 10, data_cast=np.flo
     if cache and os.path.exists(cache):
         load_table(cache)
     desc, keep, _ = build_desc(kgrid, medium, source, sensor, data_cast=data_cast, **kwargs)
-    _lib.check(lib.kw_autotune(_lib.context(device), C.byref(desc), int(steps)))
+    _lib.check(lib.kw_autotune(_lib.context(device), C.byref(desc), 1, int(steps)))
+    del keep
+    if cache:
+        save_table(cache)
+    return table()
+
+
+def autotune_batch(sims: Sequence[Tuple], *, steps: int = 10, data_cast=np.float32, device: int = 0,
+                   cache: str | None = None, **kwargs) -> Dict[Tuple[int, str, int], int]:
+    """Time the launch shapes of a batch of (kgrid, medium, source, sensor) run together.
+
+    The trials build and run the whole batch, so the table holds the shapes that
+    are fastest when the simulations share each launch (kspace_first_order_batch).
+    """
+    lib = _lib.load()
+    if cache and os.path.exists(cache):
+        load_table(cache)
+    descs, keep = [], []
+    for kgrid, medium, source, sensor in sims:
+        d, k, _ = build_desc(kgrid, medium, source, sensor, data_cast=data_cast, **kwargs)
+        descs.append(d)
+        keep.append(k)
+    arr = (_lib.SimDesc * len(descs))(*descs)
+    _lib.check(lib.kw_autotune(_lib.context(device), arr, len(descs), int(steps)))
     del keep
     if cache:
         save_table(cache)

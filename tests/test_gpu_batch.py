@@ -87,3 +87,21 @@ def test_autotune_fills_table():
     print(f"    table {t}", flush=True)
     for key in [(4, "dir_x", 64), (4, "dir_y", 96), (4, "spec_y", 96), (4, "c2r_x", 64)]:
         assert key in t and t[key] >= 1
+
+
+def test_autotune_batch_then_parity():
+    print("\n[case] batch autotune (whole-batch trials) then f64 parity", flush=True)
+    from kwave_amd import tuning
+    sims, kw = _sims([("hetc_absorb_psrc", (64, 64)), ("stokes_p0_psrc", (64, 96)), ("psrc", (128, 48))], nt=24)
+    tuning.clear_table()
+    try:
+        t = tuning.autotune_batch([tuple(copy.deepcopy(s)) for s in sims], steps=3, data_cast=np.float64, **kw)
+        print(f"    table {t}", flush=True)
+        assert (8, "dir_y", 64) in t and (8, "spec_y", 96) in t
+        outs = K.kspace_first_order_batch([tuple(copy.deepcopy(s)) for s in sims], data_cast=np.float64, **kw)
+    finally:
+        tuning.clear_table()
+    for i, (kg, med, src, sen) in enumerate(sims):
+        ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
+        for k in outs[i].keys():
+            assert rel_l2(outs[i][k], ref[k]) <= 1e-10, (i, k)
