@@ -138,6 +138,11 @@ typedef struct kw_sim_desc {
   int32_t freq_response;            /* frequency_response set */
   double freq_center;               /* [Hz] */
   double freq_bandwidth;            /* [Hz] */
+
+  /* kspace_first_order_as (axisymmetric.jl:26-227): 2-D only, x axial, y
+   * radial with r = |y_vec| (0 replaced by dy/2); the radial density update
+   * adds ur/r.  tr_data is ignored (that solver has no time reversal). */
+  int32_t axisymmetric;
 } kw_sim_desc;
 
 /* Library identity and errors. */

@@ -395,6 +395,7 @@ class Sim final : public SimBase {
     HIPC(hipSetDevice(dev_));
     if (!het_c_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient needs a sound-speed map");
     if (nonlinear_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient supports the linear equation of state");
+    if (axisym_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient is not available for the axisymmetric solver");
     if (stokes_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient needs the heterogeneous equation-of-state branch");
     if (maps_f16_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient needs maps in the kernel precision");
     if (tr_ || (psrc_[0].any && p_dir_)) raise(KW_ERR_UNSUPPORTED, "time reversal / Dirichlet sources");
@@ -683,7 +684,9 @@ class Sim final : public SimBase {
     dt_ = d.dt;
     Nt_ = d.Nt;
     Nt_alloc_ = d.Nt;
-    tr_ = d.tr_data != nullptr;
+    axisym_ = d.axisymmetric != 0;
+    if (axisym_ && nd_ != 2) raise(KW_ERR_INVALID, "the axisymmetric solver needs a 2-D grid (axial x, radial y)");
+    tr_ = d.tr_data != nullptr && !axisym_;   // axisymmetric.jl has no time-reversal branch
     HIPC(hipStreamCreate(&stream_));
 
     const bool het_c = d.c0 != nullptr, het_r = d.rho0 != nullptr;
@@ -1009,6 +1012,16 @@ class Sim final : public SimBase {
 
   void build_args(const kw_sim_desc& d) {
     const bool het_r = d.rho0 != nullptr;
+    if (axisym_) {
+      // r = |y_vec|, y_vec = (j - (Ny-1)/2) dy, 0 replaced by dy/2 (axisymmetric.jl:103-106)
+      std::vector<double> ir(N_[1]);
+      for (long long j = 0; j < N_[1]; ++j) {
+        double r = std::fabs(((double)j - (double)(N_[1] - 1) / 2.0) * d.d[1]);
+        if (r == 0.0) r = d.d[1] / 2.0;
+        ir[j] = 1.0 / r;
+      }
+      upload(inv_r_, to_t<T>(ir));
+    }
     for (int a = 0; a < nd_; ++a) {
       PassArgs<T>& A = pa_[a];
       std::memset(&A, 0, sizeof(A));
@@ -1028,6 +1041,7 @@ class Sim final : public SimBase {
       A.dt = (T)d.dt;
       A.Nx = (int)N_[0];
       A.n_lines = (int)(N_[1] * N_[2]);
+      A.inv_r = (axisym_ && a == 1) ? inv_r_.as<T>() : nullptr;
       const PassCfg& c = cfg_[a];
       A.x_blocks = (int)(((N_[0] + 1) / 2 + c.lpw - 1) / c.lpw);
       if (a == 1) {
@@ -1367,6 +1381,8 @@ class Sim final : public SimBase {
   DevBuf tr_vals_;
   // adjoint
   bool het_c_ = false, nonlinear_ = false, has_p0_ = false, absorbing_ = false, no_record_ = false;
+  bool axisym_ = false;
+  DevBuf inv_r_;
   double c_ref_ = 0, tau_ = 0, y_ = 0, anp_ = 0, d_[3] = {1, 1, 1};
   std::vector<long long> psrc_idx_h_;
   std::vector<double> psrc_sig_h_, psrc_fac_h_;
@@ -1405,7 +1421,7 @@ class Batch final : public BatchBase {
     for (int i = 0; i < n; ++i) {
       if (descs[i].ndim != descs[0].ndim) raise(KW_ERR_INVALID, "a batch needs one dimensionality");
       if (descs[i].Nt != descs[0].Nt) raise(KW_ERR_INVALID, "a batch needs one Nt");
-      if (descs[i].tr_data) raise(KW_ERR_UNSUPPORTED, "time reversal is not batched");
+      if (descs[i].tr_data && !descs[i].axisymmetric) raise(KW_ERR_UNSUPPORTED, "time reversal is not batched");
       sims_.emplace_back(new Sim<T>(device, descs[i], true));
     }
     Nt_ = descs[0].Nt;

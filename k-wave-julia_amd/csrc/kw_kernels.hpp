@@ -83,6 +83,7 @@ struct PassArgs {
   T inv_dt;
   int nhp;             // half-spectrum row pitch (Nx/2+1 rounded up to 16 complex)
   T* rho_sum_out;      // if set, the x pass also stores sum_d rho_d (adjoint recompute)
+  const T* inv_r;      // axisymmetric radial pass: 1/r along the line (else null)
 };
 
 // ---------------------------------------------------------------------------
@@ -359,6 +360,19 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dm[L.n(q)]);
   line_fft<T, N, true>(v, lds, tw, L.t);
+  if constexpr (!CONTIG) {
+    // axisymmetric radial pass: du_r/dr + u_r / r, u_r as stored above (an L2 hit)
+    if (a.inv_r != nullptr) {
+      Cx<T> ur[E];
+      L.load(a.u, ur);
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const T ir = a.inv_r[L.n(q)];
+        v[q].x += ur[q].x * ir;
+        v[q].y += ur[q].y * ir;
+      }
+    }
+  }
 
   // --- density update  rho = pml (pml rho - dt rho0 du); rho0 is re-read
   // here (an L2 hit) rather than held in registers across the transforms

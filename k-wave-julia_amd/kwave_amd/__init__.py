@@ -11,7 +11,7 @@ from .model import (AbstractKWaveGrid, Additive, AdditiveNoCorrection, Dirichlet
                     is_homogeneous, is_lossless, is_nonlinear, is_time_reversal, make_time,
                     validate_record_fields)
 from .builders import make_ball, make_bowl, make_circle, make_disc, tone_burst
-from .solver import Simulation, build_desc, kspace_first_order, quantize_map
+from .solver import Simulation, build_desc, kspace_first_order, kspace_first_order_as, quantize_map
 from .batch import Batch, kspace_first_order_batch
 from .adjoint import gradient_c0, misfit_rrule
 from . import tuning
@@ -22,6 +22,6 @@ __all__ = [
     "KWaveMedium", "KWaveSource", "KWaveSensor", "SimulationOutput", "SourceMode", "Dirichlet",
     "Additive", "AdditiveNoCorrection", "VALID_RECORD_FIELDS", "validate_record_fields",
     "is_homogeneous", "is_lossless", "is_nonlinear", "is_time_reversal", "has_p0",
-    "has_pressure_source", "has_velocity_source", "kspace_first_order", "Simulation", "build_desc", "quantize_map",
+    "has_pressure_source", "has_velocity_source", "kspace_first_order", "kspace_first_order_as", "Simulation", "build_desc", "quantize_map",
     "Batch", "kspace_first_order_batch", "gradient_c0", "misfit_rrule", "tuning", "tone_burst", "make_disc", "make_circle", "make_ball", "make_bowl", "KWaveHIPError", "LIB_PATH",
 ]

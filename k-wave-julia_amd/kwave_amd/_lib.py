@@ -94,6 +94,7 @@ class SimDesc(C.Structure):
         ("freq_response", C.c_int32),
         ("freq_center", C.c_double),
         ("freq_bandwidth", C.c_double),
+        ("axisymmetric", C.c_int32),
     ]
 
 

@@ -156,16 +156,20 @@ def quantize_map(a, storage=np.float16):
 
 
 def build_desc(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSource, sensor: KWaveSensor, *,
-               pml_size=20, pml_alpha=2.0, smooth_p0: bool = True, data_cast=np.float64, medium_storage=None):
+               pml_size=20, pml_alpha=2.0, smooth_p0: bool = True, data_cast=np.float64, medium_storage=None,
+               axisymmetric: bool = False):
     """Validate like first_order.jl:311-315/529-533 and fill kw_sim_desc.
 
     Returns (desc, keep, info); ``keep`` holds the host buffers the
-    descriptor points to (borrowed by kw_sim_create only).
+    descriptor points to (borrowed by kw_sim_create only).  ``axisymmetric``
+    selects the kspace_first_order_as loop (2-D only; no time reversal).
     """
     if kgrid.Nt == 0 or kgrid.dt == 0:
         raise ValueError("Time array not set. Call make_time!(kgrid, sound_speed) before running.")
     validate_record_fields(sensor)
     nd = kgrid.ndim
+    if axisymmetric and nd != 2:
+        raise TypeError("kspace_first_order_as takes a 2-D grid (x axial, y radial)")
     shape = kgrid.grid_size()
     nvox = int(np.prod(shape))
     d = _lib.SimDesc()
@@ -231,7 +235,8 @@ def build_desc(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSourc
         d.alpha_coeff = float(medium.alpha_coeff)
         d.alpha_power = float(medium.alpha_power)
 
-    tr = is_time_reversal(sensor)
+    d.axisymmetric = 1 if axisymmetric else 0
+    tr = is_time_reversal(sensor) and not axisymmetric      # axisymmetric.jl has no time reversal
     if tr:
         trd = np.asarray(sensor.time_reversal_boundary_data, dtype=np.float64)
         Nt = int(trd.shape[1])                                          # first_order.jl:430-431
@@ -339,6 +344,30 @@ def kspace_first_order(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KW
         raise NotImplementedError("save_to_disk (HDF5 input writer) is not part of this build")
     desc, keep, info = build_desc(kgrid, medium, source, sensor, pml_size=pml_size, pml_alpha=pml_alpha,
                                   smooth_p0=smooth_p0, data_cast=data_cast, medium_storage=medium_storage)
+    return _run(desc, keep, info, kgrid, sensor, progress_callback, device)
+
+
+def kspace_first_order_as(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSource,
+                          sensor: KWaveSensor, *, pml_inside: bool = True, pml_size=20, pml_alpha=2.0,
+                          smooth_p0: bool = True, smooth_c0: bool = False, smooth_rho0: bool = False,
+                          data_cast=np.float64, plot_sim: bool = False, plot_layout: str = "default",
+                          plot_scale="auto", record_movie: Optional[str] = None,
+                          progress_callback: Optional[Callable] = None, medium_storage=None,
+                          device: int = 0) -> SimulationOutput:
+    """Axisymmetric k-space PSTD on the GPU (axisymmetric.jl:26-227).
+
+    A 2-D grid with x axial and y radial (r = |y|); the radial density update
+    carries the ur/r term of the cylindrical divergence.  Keywords as in the
+    reference (``pml_inside``, ``smooth_c0``, ``smooth_rho0``, ``plot_*``,
+    ``record_movie`` accepted and unused).
+    """
+    desc, keep, info = build_desc(kgrid, medium, source, sensor, pml_size=pml_size, pml_alpha=pml_alpha,
+                                  smooth_p0=smooth_p0, data_cast=data_cast, medium_storage=medium_storage,
+                                  axisymmetric=True)
+    return _run(desc, keep, info, kgrid, sensor, progress_callback, device)
+
+
+def _run(desc, keep, info, kgrid, sensor, progress_callback, device) -> SimulationOutput:
     Nt = info["Nt"]
     with Simulation(desc, keep, device) as sim:
         if progress_callback is None:

@@ -426,12 +426,14 @@ def _norm_tuple(v, nd, cast):
 
 def kspace_first_order(kgrid, medium, source, sensor, *, pml_size=20, pml_alpha=2.0,
                        smooth_p0=True, data_cast=np.float64, progress_callback=None,
-                       n_steps: Optional[int] = None, **_ignored) -> Dict[str, np.ndarray]:
+                       n_steps: Optional[int] = None, axisymmetric: bool = False,
+                       **_ignored) -> Dict[str, np.ndarray]:
     """Restatement of kspace_first_order 1D/2D/3D (first_order.jl:92-281, 287-499, 505-728).
 
     ``kgrid``/``medium``/``source``/``sensor`` are duck-typed objects with the
     reference's field names.  ``n_steps`` (oracle-only) truncates the loop for
-    timing samples.  Returns the sensor-data dict (Float64 arrays).
+    timing samples.  ``axisymmetric`` selects the kspace_first_order_as loop
+    (see that function).  Returns the sensor-data dict (Float64 arrays).
     """
     g = grid_from(kgrid)
     nd = g.ndim
@@ -444,6 +446,10 @@ def kspace_first_order(kgrid, medium, source, sensor, *, pml_size=20, pml_alpha=
         if f not in VALID_RECORD_FIELDS:
             raise ValueError(f"Invalid sensor record field: :{f}")
     tr_data = getattr(sensor, "time_reversal_boundary_data", None)
+    if axisymmetric:
+        if nd != 2:
+            raise ValueError("kspace_first_order_as needs a 2-D grid (axial x, radial y)")
+        tr_data = None                 # axisymmetric.jl has no time-reversal branch
     is_tr = tr_data is not None
     dt, Nt = g.dt, g.Nt
 
@@ -470,6 +476,11 @@ def kspace_first_order(kgrid, medium, source, sensor, *, pml_size=20, pml_alpha=
 
     pmlb = [pml[a].reshape(bshape(a)) for a in range(nd)]
     pmlsgb = [pml_sg[a].reshape(bshape(a)) for a in range(nd)]
+    inv_r = None
+    if axisymmetric:                                                          # axisymmetric.jl:103-106
+        r_vec = np.abs(make_spatial_vec(shape[1], g.d[1]))
+        r_vec[r_vec == 0] = g.d[1] / 2
+        inv_r = (1.0 / r_vec).reshape(1, -1)
     rho0_sg = None
     if np.ndim(rho0) != 0:
         rho0_sg = [stagger_density(rho0, a).astype(T) for a in range(nd)]
@@ -583,6 +594,8 @@ def kspace_first_order(kgrid, medium, source, sensor, *, pml_size=20, pml_alpha=
                 tc = min(t, s.shape[1])
                 set_at(u[a], u_src_idx, s[:, tc - 1].astype(T), add=(u_mode != DIRICHLET))
         du = [spectral_gradient(u[a], g.k_vecs[a], g.shifts[a][1], a) for a in range(nd)]
+        if inv_r is not None:                                                 # axisymmetric.jl:179-187
+            du[1] = du[1] + u[1] * inv_r
         for a in range(nd):
             rho[a][...] = pmlb[a] * (pmlb[a] * rho[a] - dt * rho0 * du[a])
         if has_p_src:
@@ -616,6 +629,17 @@ def kspace_first_order(kgrid, medium, source, sensor, *, pml_size=20, pml_alpha=
     apply_frequency_response(data, sensor, g)
     data["_state"] = dict(p=p, u=u, rho=rho)   # oracle-only: final fields for operator tests
     return data
+
+
+def kspace_first_order_as(kgrid, medium, source, sensor, **kwargs) -> Dict[str, np.ndarray]:
+    """Restatement of kspace_first_order_as (axisymmetric.jl:26-227).
+
+    The 2-D loop with x axial and y radial, r = |y_vec| (0 replaced by dy/2,
+    axisymmetric.jl:103-106), and the radial density update
+    rho_r <- pml_r (pml_r rho_r - dt rho0 (d ur/dr + ur / r)) (179-187); no
+    time-reversal branch.  Same keywords as kspace_first_order.
+    """
+    return kspace_first_order(kgrid, medium, source, sensor, axisymmetric=True, **kwargs)
 
 
 def _record(data, record, p, u, idx, t, Nt):
