@@ -183,12 +183,39 @@ def bench_adjoint(a):
     obs = [K.kspace_first_order(kg, med_true, s, r, data_cast=np.float32, **kw)["p"] for s, r in firings]
     K.gradient_c0(kg, med0, firings[0][0], firings[0][1], obs[0], checkpoint_every=50, data_cast=np.float32, **kw)
     t1 = time.perf_counter()
-    loss, g = K.gradient_c0(kg, med0, [f[0] for f in firings], [f[1] for f in firings], obs,
-                            checkpoint_every=50, data_cast=np.float32, **kw)
+    loss_total, g = K.gradient_c0(kg, med0, [f[0] for f in firings], [f[1] for f in firings], obs,
+                                  checkpoint_every=50, data_cast=np.float32, **kw)
     t2 = time.perf_counter()
     assert np.all(np.isfinite(g))
     vox = int(np.prod(kg.grid_size()))
     secs = t2 - t1
+    # accuracy (untimed): FP64 adjoint of firing 0 against central differences of the FP64
+    # forward run on `a.fd_pixels` seeded pixels (h = 1e-3 m/s, dt held), and the FP32
+    # adjoint of the same firing against the FP64 one
+    fd = None
+    if a.fd_pixels > 0:
+        s0, r0 = firings[0]
+        _, g64 = K.gradient_c0(kg, med0, s0, r0, obs[0], checkpoint_every=50, data_cast=np.float64, **kw)
+        _, g32 = K.gradient_c0(kg, med0, s0, r0, obs[0], checkpoint_every=50, data_cast=np.float32, **kw)
+
+        def loss(c):
+            m = K.KWaveMedium(sound_speed=c, density=1000.0)
+            p = K.kspace_first_order(kg, m, s0, r0, data_cast=np.float64, **kw)["p"]
+            return 0.5 * float(np.sum((p - obs[0]) ** 2))
+
+        rng = np.random.default_rng(42)
+        pix = rng.choice(vox, size=a.fd_pixels, replace=False)
+        gf, h, errs = g64.ravel(order="F"), 1e-3, []
+        for i in pix:
+            e = np.zeros(vox)
+            e[i] = h
+            e = e.reshape(c_start.shape, order="F")
+            d = (loss(c_start + e) - loss(c_start - e)) / (2 * h)
+            errs.append(abs(d - gf[i]))
+        gmax = float(np.abs(gf).max())
+        fd = {"pixels": int(a.fd_pixels), "h_m_per_s": h, "firing": 0,
+              "max_abs_err_over_max_grad": max(errs) / gmax,
+              "fp32_vs_fp64_grad_rel_l2": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64))}
     out = {
         "metric": METRIC, "value": vox * kg.Nt * len(firings) / secs, "unit": "voxel-updates/s", "n_gpus": 1,
         "steps": kg.Nt * len(firings), "warmup": kg.Nt, "ms_per_step": secs * 1e3 / (kg.Nt * len(firings)),
@@ -197,7 +224,7 @@ def bench_adjoint(a):
                                f"Nt={kg.Nt}, checkpoints every 50 steps; value counts forward time steps "
                                f"(each one forward + one recompute + one adjoint step)",
                    "parallelism": "single GPU"},
-        "seconds_per_firing": secs / len(firings), "loss": loss,
+        "seconds_per_firing": secs / len(firings), "loss": loss_total, "fd_check": fd,
     }
     print(json.dumps(out))
 
@@ -261,6 +288,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", choices=["c2", "batch", "adjoint", "c4"], default="c2")
     ap.add_argument("--firings", type=int, default=8)
+    ap.add_argument("--fd-pixels", type=int, default=32)
     ap.add_argument("--copy-probe", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
