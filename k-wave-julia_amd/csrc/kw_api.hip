@@ -759,7 +759,7 @@ class Sim final : public SimBase {
   PassCfg pass_cfg(long long n, bool contig, long long lines, long long outer, int cls) {
     PassCfg c;
     c.n = (int)n;
-    const int tpl = plan_tpl((int)n);
+    const int tpl = plan_tpl((int)n, contig ? 1 : 0);
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
     int lpw = 256 / tpl;
     while (lpw > 1 && (size_t)lpw * line_bytes > 65536) lpw /= 2;

@@ -229,11 +229,15 @@ struct DFT<T, 5, INV> {
 template <int... Rs>
 struct Radices {};
 
-template <int N>
+// V = 0: plans of the strided and spectral passes (lines spread across
+// workgroup lanes, so more lines per workgroup = wider coalesced rows);
+// V = 1: plans of the contiguous x-line passes, which may trade a larger
+// TPL (fewer registers, more waves) for an extra LDS exchange.
+template <int N, int V = 0>
 struct Plan;
 #define KW_PLAN(N_, TPL_, ...)                     \
   template <>                                      \
-  struct Plan<N_> {                                \
+  struct Plan<N_, 0> {                             \
     static constexpr int N = N_;                   \
     static constexpr int TPL = TPL_;               \
     static constexpr int E = N_ / TPL_;            \
@@ -264,6 +268,15 @@ KW_PLAN(1024, 64, 16, 16, 4)
 KW_PLAN(2048, 128, 16, 16, 8)
 KW_PLAN(4096, 256, 16, 16, 16)
 #undef KW_PLAN
+template <int N>
+struct Plan<N, 1> : Plan<N, 0> {};
+template <>
+struct Plan<256, 1> {
+  static constexpr int N = 256;
+  static constexpr int TPL = 32;
+  static constexpr int E = 8;
+  using R = Radices<8, 8, 4>;
+};
 
 // LDS slice of one line: one complex of padding per 16 keeps the
 // stride-16/32 Stockham writes off a single bank group.  The 4096 plan is
@@ -288,13 +301,13 @@ struct RadixArray {
 template <int... Rs>
 constexpr int RadixArray<Rs...>::r[];
 
-template <int N, typename RL>
+template <int N, int V, typename RL>
 struct TwCount;
-template <int N, int... Rs>
-struct TwCount<N, Radices<Rs...>> {
+template <int N, int V, int... Rs>
+struct TwCount<N, V, Radices<Rs...>> {
   static constexpr int f() {
     constexpr int rr[] = {Rs...};
-    int E = N / Plan<N>::TPL, Ns = 1, tot = 0;
+    int E = N / Plan<N, V>::TPL, Ns = 1, tot = 0;
     for (int i = 0; i < (int)sizeof...(Rs); ++i) {
       if (Ns > 1) tot += (E / rr[i]) * (rr[i] - 1);
       Ns *= rr[i];
@@ -304,17 +317,17 @@ struct TwCount<N, Radices<Rs...>> {
   static constexpr int value = f();
 };
 
-template <typename T, int N>
+template <typename T, int N, int V = 0>
 struct Twiddles {
-  static constexpr int TOT = TwCount<N, typename Plan<N>::R>::value;
+  static constexpr int TOT = TwCount<N, V, typename Plan<N, V>::R>::value;
   static constexpr bool REG = TOT > 0 && TOT <= 16;
   Cx<T> w[REG ? TOT : 1];
   const Cx<T>* g;
 };
 
-template <typename T, int N, int TPL, int Ns, int OFF, int R, int... Rest>
+template <typename T, int N, int V, int TPL, int Ns, int OFF, int R, int... Rest>
 struct TwLoad {
-  __device__ __forceinline__ static void run(Twiddles<T, N>& tw, const Cx<T>* __restrict__ table, int t) {
+  __device__ __forceinline__ static void run(Twiddles<T, N, V>& tw, const Cx<T>* __restrict__ table, int t) {
     constexpr int NB = (N / TPL) / R;
     if constexpr (Ns > 1) {
 #pragma unroll
@@ -325,33 +338,33 @@ struct TwLoad {
       }
     }
     if constexpr (sizeof...(Rest) > 0)
-      TwLoad<T, N, TPL, Ns * R, (Ns > 1 ? OFF + NB * (R - 1) : OFF), Rest...>::run(tw, table, t);
+      TwLoad<T, N, V, TPL, Ns * R, (Ns > 1 ? OFF + NB * (R - 1) : OFF), Rest...>::run(tw, table, t);
   }
 };
-template <typename T, int N, typename RL>
+template <typename T, int N, int V, typename RL>
 struct TwLoadImpl;
-template <typename T, int N, int... Rs>
-struct TwLoadImpl<T, N, Radices<Rs...>> {
-  __device__ __forceinline__ static void run(Twiddles<T, N>& tw, const Cx<T>* __restrict__ table, int t) {
-    TwLoad<T, N, Plan<N>::TPL, 1, 0, Rs...>::run(tw, table, t);
+template <typename T, int N, int V, int... Rs>
+struct TwLoadImpl<T, N, V, Radices<Rs...>> {
+  __device__ __forceinline__ static void run(Twiddles<T, N, V>& tw, const Cx<T>* __restrict__ table, int t) {
+    TwLoad<T, N, V, Plan<N, V>::TPL, 1, 0, Rs...>::run(tw, table, t);
   }
 };
 
 // Prepare the twiddles of a line of length N for thread t of the line.
-template <typename T, int N>
-__device__ __forceinline__ Twiddles<T, N> load_twiddles(const Cx<T>* __restrict__ table, int t) {
-  Twiddles<T, N> tw;
+template <typename T, int N, int V = 0>
+__device__ __forceinline__ Twiddles<T, N, V> load_twiddles(const Cx<T>* __restrict__ table, int t) {
+  Twiddles<T, N, V> tw;
   tw.g = table;
-  if constexpr (Twiddles<T, N>::REG) TwLoadImpl<T, N, typename Plan<N>::R>::run(tw, table, t);
+  if constexpr (Twiddles<T, N, V>::REG) TwLoadImpl<T, N, V, typename Plan<N, V>::R>::run(tw, table, t);
   return tw;
 }
 
-template <typename T, int N, int TPL, bool INV, int Ns, int OFF, int R, int... Rest>
+template <typename T, int N, int V, int TPL, bool INV, int Ns, int OFF, int R, int... Rest>
 struct StockhamPass {
   static constexpr int E = N / TPL;
   static constexpr int NB = E / R;
   static_assert(E % R == 0, "radix must divide the per-thread element count");
-  __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Twiddles<T, N>& tw, int t) {
+  __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Twiddles<T, N, V>& tw, int t) {
     constexpr bool LAST = sizeof...(Rest) == 0;
     static_assert(!LAST || Ns * R == N, "radix product must equal N");
 #pragma unroll
@@ -365,7 +378,7 @@ struct StockhamPass {
 #pragma unroll
         for (int r = 1; r < R; ++r) {
           Cx<T> w;
-          if constexpr (Twiddles<T, N>::REG)
+          if constexpr (Twiddles<T, N, V>::REG)
             w = tw.w[OFF + m * (R - 1) + (r - 1)];
           else
             w = tw.g[jm * r * (N / (Ns * R))];
@@ -387,28 +400,29 @@ struct StockhamPass {
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = lds[LdsLayout<N>::at(t + TPL * q)];
       __syncthreads();
-      StockhamPass<T, N, TPL, INV, Ns * R, (Ns > 1 ? OFF + NB * (R - 1) : OFF), Rest...>::run(v, lds, tw, t);
+      StockhamPass<T, N, V, TPL, INV, Ns * R, (Ns > 1 ? OFF + NB * (R - 1) : OFF), Rest...>::run(v, lds, tw, t);
     }
   }
 };
 
-template <typename T, int N, bool INV, typename RL>
+template <typename T, int N, int V, bool INV, typename RL>
 struct LineFFTImpl;
-template <typename T, int N, bool INV, int... Rs>
-struct LineFFTImpl<T, N, INV, Radices<Rs...>> {
-  static constexpr int TPL = Plan<N>::TPL;
-  static constexpr int E = Plan<N>::E;
-  __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Twiddles<T, N>& tw, int t) {
-    StockhamPass<T, N, TPL, INV, 1, 0, Rs...>::run(v, lds, tw, t);
+template <typename T, int N, int V, bool INV, int... Rs>
+struct LineFFTImpl<T, N, V, INV, Radices<Rs...>> {
+  static constexpr int TPL = Plan<N, V>::TPL;
+  static constexpr int E = Plan<N, V>::E;
+  __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Twiddles<T, N, V>& tw, int t) {
+    StockhamPass<T, N, V, TPL, INV, 1, 0, Rs...>::run(v, lds, tw, t);
   }
 };
 
 // Public entry: forward (INV=false) or unnormalised inverse transform of the
 // line held in v (stride-TPL layout), using the line's LDS slice `lds`.
 // Every thread of the workgroup must call it (it contains barriers).
-template <typename T, int N, bool INV>
-__device__ __forceinline__ void line_fft(Cx<T> (&v)[Plan<N>::E], Cx<T>* lds, const Twiddles<T, N>& tw, int t) {
-  LineFFTImpl<T, N, INV, typename Plan<N>::R>::run(v, lds, tw, t);
+template <typename T, int N, bool INV, int V = 0>
+__device__ __forceinline__ void line_fft(Cx<T> (&v)[(Plan<N, V>::E)], Cx<T>* lds, const Twiddles<T, N, V>& tw,
+                                         int t) {
+  LineFFTImpl<T, N, V, INV, typename Plan<N, V>::R>::run(v, lds, tw, t);
 }
 
 }  // namespace kw

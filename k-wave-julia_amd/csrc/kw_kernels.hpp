@@ -119,10 +119,10 @@ __device__ __forceinline__ U& at32(U* base, unsigned i) {
 
 // Addresses are a per-workgroup 64-bit base (uniform) plus 32-bit lane
 // offsets, so every load is a scalar-base + vector-offset access.
-template <typename T, int N>
+template <typename T, int N, int V = 0>
 struct LineCtx {
-  static constexpr int TPL = Plan<N>::TPL;
-  static constexpr int E = Plan<N>::E;
+  static constexpr int TPL = Plan<N, V>::TPL;
+  static constexpr int E = Plan<N, V>::E;
   using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
   int t, c;
   bool va, vb;          // real line a / b valid
@@ -213,9 +213,10 @@ struct LineCtx {
 };
 
 template <typename T, int N, bool CONTIG>
-__device__ __forceinline__ LineCtx<T, N> make_ctx(const PassArgs<T>& a, int wg, int& lpw_out) {
-  constexpr int TPL = Plan<N>::TPL;
-  LineCtx<T, N> L;
+__device__ __forceinline__ LineCtx<T, N, CONTIG ? 1 : 0> make_ctx(const PassArgs<T>& a, int wg, int& lpw_out) {
+  constexpr int V = CONTIG ? 1 : 0;
+  constexpr int TPL = Plan<N, V>::TPL;
+  LineCtx<T, N, V> L;
   const int tid = threadIdx.x;
   const int lpw = blockDim.x / TPL;
   lpw_out = lpw;
@@ -250,13 +251,13 @@ __device__ __forceinline__ LineCtx<T, N> make_ctx(const PassArgs<T>& a, int wg, 
 
 // Apply the workgroup's source entries to the pair-line registers through LDS.
 // pos packs (c << 21) | (part << 20) | n.
-template <typename T, int N>
-__device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)[Plan<N>::E], Cx<T>* lds_all,
-                                       const LineCtx<T, N>& L, int wg) {
+template <typename T, int N, int V>
+__device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)[(Plan<N, V>::E)], Cx<T>* lds_all,
+                                       const LineCtx<T, N, V>& L, int wg) {
   if (s.off == nullptr) return;
   const int e0 = s.off[wg], e1 = s.off[wg + 1];
   if (e0 >= e1) return;  // uniform across the workgroup
-  constexpr int E = Plan<N>::E;
+  constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   Cx<T>* lds = lds_all + L.c * P;
 #pragma unroll
@@ -286,22 +287,23 @@ __device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)
 // for that path.
 template <typename T, int N, bool CONTIG, bool TWO = false>
 __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int t_index) {
-  constexpr int E = Plan<N>::E;
+  constexpr int V = CONTIG ? 1 : 0;  // plan variant of contiguous x-line passes
+  constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
   int lpw;
-  const LineCtx<T, N> L = make_ctx<T, N, CONTIG>(a, wg, lpw);
+  const LineCtx<T, N, V> L = make_ctx<T, N, CONTIG>(a, wg, lpw);
   Cx<T>* lds = lds_all + L.c * P;
-  const Twiddles<T, N> tw = load_twiddles<T, N>(a.tw, L.t);
+  const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(a.tw, L.t);
 
   // --- grad p (staggered +) : rfft -> i k e^{+ikh/2} -> irfft
   Cx<T> v[E];
   L.load(a.p, v);
-  line_fft<T, N, false>(v, lds, tw, L.t);
+  line_fft<T, N, false, V>(v, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dp[L.n(q)]);
-  line_fft<T, N, true>(v, lds, tw, L.t);
+  line_fft<T, N, true, V>(v, lds, tw, L.t);
 
   // --- velocity update  u = pml_sg (pml_sg u - dt/rho0_sg dp)
   Cx<T> r0[E];
@@ -352,14 +354,14 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
     L.store(a.u, v);
     return;
   }
-  inject<T, N>(a.usrc, t_index, v, lds_all, L, wg);
+  inject<T, N, V>(a.usrc, t_index, v, lds_all, L, wg);
   L.store(a.u, v);
 
   // --- div u (staggered -)
-  line_fft<T, N, false>(v, lds, tw, L.t);
+  line_fft<T, N, false, V>(v, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dm[L.n(q)]);
-  line_fft<T, N, true>(v, lds, tw, L.t);
+  line_fft<T, N, true, V>(v, lds, tw, L.t);
   if constexpr (!CONTIG) {
     // axisymmetric radial pass: du_r/dr + u_r / r, u_r as stored above (an L2 hit)
     if (a.inv_r != nullptr) {
@@ -399,7 +401,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = w[q];
   }
-  inject<T, N>(a.psrc, t_index, v, lds_all, L, wg);
+  inject<T, N, V>(a.psrc, t_index, v, lds_all, L, wg);
   L.store(a.rho, v);
 
   if constexpr (CONTIG) {
@@ -495,7 +497,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
       Cx<T> z[E];
 #pragma unroll
       for (int q = 0; q < E; ++q) z[q] = s == 0 ? f[q] : g[TWO ? q : 0];
-      line_fft<T, N, false>(z, lds, tw, L.t);
+      line_fft<T, N, false, V>(z, lds, tw, L.t);
 #pragma unroll
       for (int q = 0; q < E; ++q) lds[LdsLayout<N>::at(L.n(q))] = z[q];
       __syncthreads();
@@ -711,8 +713,9 @@ struct C2RArgs {
 
 template <typename T, int N>
 __device__ __forceinline__ void c2r_body(const C2RArgs<T>& a, int wg) {
-  constexpr int TPL = Plan<N>::TPL;
-  constexpr int E = Plan<N>::E;
+  constexpr int V = 1;  // contiguous x lines
+  constexpr int TPL = Plan<N, V>::TPL;
+  constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   constexpr int NH = N / 2 + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -723,7 +726,7 @@ __device__ __forceinline__ void c2r_body(const C2RArgs<T>& a, int wg) {
   const long long la = 2 * pi, lb = 2 * pi + 1;
   const bool va = la < a.n_lines, vb = lb < a.n_lines;
   Cx<T>* lds = lds_all + c * P;
-  const Twiddles<T, N> tw = load_twiddles<T, N>(a.tw, t);
+  const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(a.tw, t);
   Cx<T> z[E];
 #pragma unroll
   for (int q = 0; q < E; ++q) {
@@ -755,7 +758,7 @@ __device__ __forceinline__ void c2r_body(const C2RArgs<T>& a, int wg) {
     }
     z[q] = Cx<T>{A.x - B.y, A.y + B.x};
   }
-  line_fft<T, N, true>(z, lds, tw, t);
+  line_fft<T, N, true, V>(z, lds, tw, t);
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int n = t + TPL * q;
@@ -1062,7 +1065,8 @@ struct AdjArgs {
 
 template <typename T, int N, bool CONTIG>
 __global__ void __launch_bounds__(256) k_adj_pass(AdjArgs<T> a) {
-  constexpr int E = Plan<N>::E;
+  constexpr int V = CONTIG ? 1 : 0;
+  constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
@@ -1073,9 +1077,9 @@ __global__ void __launch_bounds__(256) k_adj_pass(AdjArgs<T> a) {
   geo.n_lines = a.n_lines;
   geo.x_blocks = a.x_blocks;
   int lpw;
-  const LineCtx<T, N> L = make_ctx<T, N, CONTIG>(geo, blockIdx.x, lpw);
+  const LineCtx<T, N, V> L = make_ctx<T, N, CONTIG>(geo, blockIdx.x, lpw);
   Cx<T>* lds = lds_all + L.c * P;
-  const Twiddles<T, N> tw = load_twiddles<T, N>(a.tw, L.t);
+  const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(a.tw, L.t);
 
   Cx<T> rt[E], cc[E], v[E];
   L.load(a.Rb, rt);
@@ -1101,10 +1105,10 @@ __global__ void __launch_bounds__(256) k_adj_pass(AdjArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cscale(rt[q], f);
   }
-  line_fft<T, N, false>(v, lds, tw, L.t);
+  line_fft<T, N, false, V>(v, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dp[L.n(q)]);
-  line_fft<T, N, true>(v, lds, tw, L.t);
+  line_fft<T, N, true, V>(v, lds, tw, L.t);
   // Ut = Ubar + D+(...) ; y = Psg Ut ; Ubar <- Psg y ; z = dt/rho0sg y
   {
     Cx<T> ub[E];
@@ -1131,10 +1135,10 @@ __global__ void __launch_bounds__(256) k_adj_pass(AdjArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cscale(v[q], f);
   }
-  line_fft<T, N, false>(v, lds, tw, L.t);
+  line_fft<T, N, false, V>(v, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dm[L.n(q)]);
-  line_fft<T, N, true>(v, lds, tw, L.t);
+  line_fft<T, N, true, V>(v, lds, tw, L.t);
   if (a.accumulate) {
     Cx<T> pb[E];
     L.load(a.Pb, pb);
@@ -1147,7 +1151,8 @@ __global__ void __launch_bounds__(256) k_adj_pass(AdjArgs<T> a) {
 // Real lines (pairs) of f -> half spectrum along x (the r2c stage alone).
 template <typename T, int N>
 __global__ void __launch_bounds__(256) k_r2c_x(const T* f, Cx<T>* S, int n_lines, int nhp, const Cx<T>* twt) {
-  constexpr int E = Plan<N>::E;
+  constexpr int V = 1;  // contiguous x lines
+  constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   constexpr int NH = N / 2 + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1155,12 +1160,12 @@ __global__ void __launch_bounds__(256) k_r2c_x(const T* f, Cx<T>* S, int n_lines
   PassArgs<T> geo;
   geo.n_lines = n_lines;
   int lpw;
-  const LineCtx<T, N> L = make_ctx<T, N, true>(geo, blockIdx.x, lpw);
+  const LineCtx<T, N, V> L = make_ctx<T, N, true>(geo, blockIdx.x, lpw);
   Cx<T>* lds = lds_all + L.c * P;
-  const Twiddles<T, N> tw = load_twiddles<T, N>(twt, L.t);
+  const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(twt, L.t);
   Cx<T> z[E];
   L.load(f, z);
-  line_fft<T, N, false>(z, lds, tw, L.t);
+  line_fft<T, N, false, V>(z, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) lds[LdsLayout<N>::at(L.n(q))] = z[q];
   __syncthreads();

@@ -14,11 +14,11 @@ bool size_supported(long long n) {
   }
 }
 
-int plan_tpl(int n) {
+int plan_tpl(int n, int variant) {
   switch (n) {
 #define X(n_) \
   case n_:    \
-    return Plan<n_>::TPL;
+    return variant ? Plan<n_, 1>::TPL : Plan<n_, 0>::TPL;
     KW_SIZES(X)
 #undef X
   }
