@@ -10,7 +10,10 @@ from .model import (AbstractKWaveGrid, Additive, AdditiveNoCorrection, Dirichlet
                     SourceMode, VALID_RECORD_FIELDS, has_p0, has_pressure_source, has_velocity_source,
                     is_homogeneous, is_lossless, is_nonlinear, is_time_reversal, make_time,
                     validate_record_fields)
-from .builders import make_ball, make_bowl, make_circle, make_disc, tone_burst
+from .builders import make_arc, make_ball, make_bowl, make_circle, make_disc, tone_burst
+from .karray import (ArcElement, BowlElement, DiscElement, KWaveArray, RectElement, add_arc_element,
+                     add_bowl_element, add_disc_element, add_rect_element, combine_sensor_data,
+                     get_array_binary_mask, get_distributed_source_signal, get_element_binary_mask)
 from .solver import Simulation, build_desc, kspace_first_order, kspace_first_order_as, quantize_map
 from .batch import Batch, kspace_first_order_batch
 from .adjoint import gradient_c0, misfit_rrule
@@ -23,5 +26,8 @@ __all__ = [
     "Additive", "AdditiveNoCorrection", "VALID_RECORD_FIELDS", "validate_record_fields",
     "is_homogeneous", "is_lossless", "is_nonlinear", "is_time_reversal", "has_p0",
     "has_pressure_source", "has_velocity_source", "kspace_first_order", "kspace_first_order_as", "Simulation", "build_desc", "quantize_map",
-    "Batch", "kspace_first_order_batch", "gradient_c0", "misfit_rrule", "tuning", "tone_burst", "make_disc", "make_circle", "make_ball", "make_bowl", "KWaveHIPError", "LIB_PATH",
+    "Batch", "kspace_first_order_batch", "gradient_c0", "misfit_rrule", "tuning", "tone_burst", "make_disc", "make_circle", "make_arc", "make_ball", "make_bowl",
+    "KWaveArray", "ArcElement", "BowlElement", "DiscElement", "RectElement", "add_arc_element", "add_bowl_element",
+    "add_disc_element", "add_rect_element", "get_element_binary_mask", "get_array_binary_mask",
+    "get_distributed_source_signal", "combine_sensor_data", "KWaveHIPError", "LIB_PATH",
 ]

@@ -1,7 +1,8 @@
 """Host-side input builders the configurations use (no GPU work here).
 
 tone_burst (signal/generation.jl:30-71), make_disc / make_circle
-(geometry/shapes.jl:22-107), make_ball (260-278), make_bowl (330-377).
+(geometry/shapes.jl:22-107), make_arc (130-168), make_ball (260-278),
+make_bowl (330-377).
 Julia's ``round(Int, x)`` rounds ties to even, like Python's ``round``.
 """
 from __future__ import annotations
@@ -61,6 +62,27 @@ def make_circle(Nx: int, Ny: int, cx, cy, radius) -> np.ndarray:
         if 1 <= px <= Nx and 1 <= py <= Ny:
             mask[px - 1, py - 1] = True
     return mask
+
+
+def make_arc(Nx: int, Ny: int, cx, cy, radius, diameter, focus) -> np.ndarray:
+    """shapes.jl:130-168 -- single-pixel arc sampled at max(200, ...) angles around
+    the centre of curvature (cx, cy), opening towards `focus` (1-based indices)."""
+    cx, cy = int(round(cx)), int(round(cy))
+    fx, fy = int(round(focus[0])), int(round(focus[1]))
+    arc = np.zeros((Nx, Ny), dtype=bool)
+    dx, dy = fx - cx, fy - cy
+    if math.isclose(math.sqrt(dx * dx + dy * dy), 0.0, abs_tol=1e-12):
+        return arc
+    center_angle = math.atan2(dy, dx)
+    half = math.asin(min(max(diameter / (2 * radius), -1.0), 1.0))
+    n_points = max(200, int(math.ceil(4 * radius * 2 * half / (2 * math.pi))))
+    for i in range(n_points + 1):
+        th = center_angle - half + i * 2 * half / n_points
+        px = int(np.rint(cx + radius * math.cos(th)))
+        py = int(np.rint(cy + radius * math.sin(th)))
+        if 1 <= px <= Nx and 1 <= py <= Ny:
+            arc[px - 1, py - 1] = True
+    return arc
 
 
 def make_ball(Nx: int, Ny: int, Nz: int, cx, cy, cz, radius) -> np.ndarray:
