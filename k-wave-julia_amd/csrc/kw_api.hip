@@ -616,7 +616,7 @@ class Sim final : public SimBase {
             n1 = tau_ * std::pow(kk, y_);
             dn1 = -2.0 * anp_ * (y_ - 1.0) * std::pow(c_ref_, y_ - 2.0) * std::pow(kk, y_);
           }
-          m[ix + hx * (iy + hy * iz)] = (T)((dkap * (1.0 + n1) + kap * dn1) * inv_n);
+          m[ix + hx * (iz + hz * iy)] = (T)((dkap * (1.0 + n1) + kap * dn1) * inv_n);
         }
     upload(out, m);
   }
@@ -839,7 +839,9 @@ class Sim final : public SimBase {
             v1 = c2 * kap * (1.0 + n1);
             if (stokes_) v2 = -c2 * kap * (eta * std::pow(kk, y - 1.0));
           }
-          const long long i = ix + hx * (iy + hy * iz);
+          // layout [|ky|][|kz|][kx]: the multiplying pass (z lines in 3-D, y lines
+          // in 2-D) reads each table row as one compact block per outer index
+          const long long i = ix + hx * (iz + hz * iy);
           m1[i] = (T)(v1 * inv_n);
           if (stokes_) m2[i] = (T)(v2 * inv_n);
         }
@@ -1118,7 +1120,7 @@ class Sim final : public SimBase {
       Z.Ny = (int)N_[1];
       Z.Nz = (int)N_[2];
       Z.MH1 = (int)NH_;
-      Z.MH2 = hy;
+      Z.MH2 = hz;
     }
     std::memset(&ca_, 0, sizeof(ca_));
     ca_.S1 = S1_.as<Cx<T>>();

@@ -554,10 +554,10 @@ struct SpecArgs {
   int x_blocks;
   int mode;           // 0 fwd, 1 inv, 2 fwd*M*inv, 3 fwd*W*inv (1-D window W along the line)
   const T* W;         // mode 3 window (length N, 1/N folded in)
-  // octant table addressing: index = kx + MH1*(|i1| + MH2*|i2|)
+  // octant table addressing: index = kx + MH1*(|line index| + MH2*|outer index|)
   int axis;           // 1 = line along y, 2 = line along z
   int Ny, Nz;
-  int MH1, MH2;       // Ny/2+1, Nz/2+1 (octant extents)
+  int MH1, MH2;       // Nx/2+1 and the octant extent of the line axis
 };
 
 // MODE: 0 forward (both spectra when S2 is set), 1 inverse, 2 fwd*M1*inv,
@@ -623,8 +623,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
     for (int q = 0; q < E; ++q) {
       int ik = t + TPL * q;
       ik = (ik > N - ik) ? N - ik : ik;
-      const unsigned mi = a.axis == 1 ? (unsigned)(kx + a.MH1 * (ik + a.MH2 * io))
-                                      : (unsigned)(kx + a.MH1 * (io + a.MH2 * ik));
+      const unsigned mi = (unsigned)(kx + a.MH1 * (ik + a.MH2 * io));
       const T m1 = valid ? at32(a.M1, mi) : (T)0;
       const T m2 = valid ? at32(a.M2, mi) : (T)0;
       w[q] = cscale(v[q], m2);
@@ -664,8 +663,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
     for (int q = 0; q < E; ++q) {
       int ik = t + TPL * q;
       ik = (ik > N - ik) ? N - ik : ik;
-      const unsigned mi = a.axis == 1 ? (unsigned)(kx + a.MH1 * (ik + a.MH2 * io))
-                                      : (unsigned)(kx + a.MH1 * (io + a.MH2 * ik));
+      const unsigned mi = (unsigned)(kx + a.MH1 * (ik + a.MH2 * io));
       const T m1 = valid ? at32(a.M1, mi) : (T)0;
       Cx<T> r = cscale(v[q], m1);
       if constexpr (MODE == 4) {
