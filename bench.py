@@ -286,6 +286,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 accuracy run")
     ap.add_argument("--config", choices=["c2", "batch", "adjoint", "c4"], default="c2")
     ap.add_argument("--firings", type=int, default=8)
     ap.add_argument("--fd-pixels", type=int, default=32)
@@ -333,7 +334,24 @@ def main():
     pmax = sim.sensor("p_max")
     if not np.all(np.isfinite(pmax)):
         raise RuntimeError("non-finite sensor data")
-    sim.close()
+    fp64_check = None
+    if not a.no_fp64 and sim.steps_done() == nt:
+        # the same kernels compiled in FP64 (parity-tested against the FP64 oracle) on the
+        # full workload, untimed: rel-L2 of the FP32 outputs against them
+        out32 = {f: sim.sensor(f) for f in ("p", "p_max", "p_rms")}
+        sim.close()
+        d64, k64, _ = K.build_desc(kg, med, src, sen, pml_size=kw["pml_size"], pml_alpha=kw["pml_alpha"],
+                                   smooth_p0=False, data_cast=np.float64)
+        with K.Simulation(d64, k64) as s64:
+            s64.run(nt)
+            s64.synchronize()
+            fp64_check = {}
+            for f, v in out32.items():
+                r = s64.sensor(f)
+                fp64_check[f] = float(np.linalg.norm(v - r) / np.linalg.norm(r))
+        fp64_check["steps"] = nt
+    else:
+        sim.close()
 
     N = a.N
     vox = N ** 3
@@ -370,6 +388,7 @@ def main():
         "step_roofline": {"bytes_per_voxel_update": step_b, "achieved": step_b * value / 1e9,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_b * value / 1e9 / HBM_PEAK_GBS},
         "kernels_ms": {k: round(v["ms"], 5) for k, v in prof.items()},
+        "fp32_vs_fp64_rel_l2": fp64_check,
         "setup_s": setup_s,
     }
     if not a.no_cpu:
