@@ -235,7 +235,7 @@ def bench_c4(a):
     import kwave_amd as K
     from kwave_amd.configs import config4
 
-    kg, med, src, sensors, kw = config4(Nt=a.warmup + a.steps)
+    kg, med, src, sensors, kw = config4(Nt=a.warmup + a.steps + a.profile_steps)
     res = {}
     for tag, sen in zip(("p_max_volume", "p_4096_points"), sensors):
         desc, keep, _ = K.build_desc(kg, med, src, sen, data_cast=np.float32, **kw)
@@ -247,11 +247,13 @@ def bench_c4(a):
         sim.run(a.steps)
         sim.synchronize()
         t2 = time.perf_counter()
+        prof = sim.profile(a.profile_steps) if a.profile_steps > 0 else {}
         field = sen.record[0]
         v = sim.sensor(field)
         assert np.all(np.isfinite(v))
         sim.close()
-        res[tag] = {"value": kg.Nx ** 3 * a.steps / (t2 - t1), "ms_per_step": (t2 - t1) * 1e3 / a.steps}
+        res[tag] = {"value": kg.Nx ** 3 * a.steps / (t2 - t1), "ms_per_step": (t2 - t1) * 1e3 / a.steps,
+                    "kernels_ms": {k: round(x["ms"], 4) for k, x in prof.items()}}
     first = res["p_max_volume"]
     out = {
         "metric": METRIC, "value": first["value"], "unit": "voxel-updates/s", "n_gpus": 1, "steps": a.steps,

@@ -261,7 +261,7 @@ KW_PLAN(192, 8, 24, 8)
 KW_PLAN(256, 16, 16, 16)
 KW_PLAN(320, 16, 20, 4, 4)
 KW_PLAN(384, 32, 12, 4, 4, 2)
-KW_PLAN(512, 64, 8, 8, 8)
+KW_PLAN(512, 32, 16, 8, 4)
 KW_PLAN(640, 32, 20, 4, 4, 2)
 KW_PLAN(768, 64, 12, 4, 4, 4)
 KW_PLAN(1024, 64, 16, 16, 4)
@@ -270,6 +270,13 @@ KW_PLAN(4096, 256, 16, 16, 16)
 #undef KW_PLAN
 template <int N>
 struct Plan<N, 1> : Plan<N, 0> {};
+template <>
+struct Plan<512, 1> {
+  static constexpr int N = 512;
+  static constexpr int TPL = 64;
+  static constexpr int E = 8;
+  using R = Radices<8, 8, 8>;
+};
 template <>
 struct Plan<256, 1> {
   static constexpr int N = 256;
