@@ -236,6 +236,8 @@ def bench_c4(a):
     from kwave_amd.configs import config4
 
     kg, med, src, sensors, kw = config4(Nt=a.warmup + a.steps + a.profile_steps)
+    if a.c4_maps == "fp32":
+        kw["medium_storage"] = None   # comparison run: maps in the kernel precision
     res = {}
     for tag, sen in zip(("p_max_volume", "p_4096_points"), sensors):
         desc, keep, _ = K.build_desc(kg, med, src, sen, data_cast=np.float32, **kw)
@@ -292,6 +294,7 @@ def main():
     ap.add_argument("--config", choices=["c2", "batch", "adjoint", "c4"], default="c2")
     ap.add_argument("--firings", type=int, default=8)
     ap.add_argument("--fd-pixels", type=int, default=32)
+    ap.add_argument("--c4-maps", choices=["fp16", "fp32"], default="fp16")
     ap.add_argument("--copy-probe", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:

@@ -678,6 +678,11 @@ class Sim final : public SimBase {
     if (d.map_storage != KW_MAPS_NATIVE && d.map_storage != KW_MAPS_FP16)
       raise(KW_ERR_INVALID, "unknown map storage mode");
     maps_f16_ = d.map_storage == KW_MAPS_FP16;
+    {
+      int v = 0;
+      HIPC(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_));
+      lds_max_ = v > 0 ? (size_t)v : 65536;
+    }
     nvox_ = N_[0] * N_[1] * N_[2];
     NH_ = N_[0] / 2 + 1;
     NHP_ = (NH_ + 15) / 16 * 16;  // 128-byte aligned spectrum rows (FP32)
@@ -761,8 +766,10 @@ class Sim final : public SimBase {
     c.n = (int)n;
     const int tpl = plan_tpl((int)n, contig ? 1 : 0);
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
-    int lpw = 256 / tpl;
-    while (lpw > 1 && (size_t)lpw * line_bytes > 65536) lpw /= 2;
+    // strided lines: at least 16 line pairs per workgroup (128-byte rows in FP32)
+    // when that fits 512 threads; contiguous lines: 256 threads
+    int lpw = (contig ? 256 : std::max(256, std::min(512, 16 * tpl))) / tpl;
+    while (lpw > 1 && (size_t)lpw * line_bytes > lds_max_) lpw /= 2;
     lpw = tuned_lpw((int)sizeof(T), cls, (int)n, lpw);
     c.req = lpw;
     // pairs that exist
@@ -771,7 +778,8 @@ class Sim final : public SimBase {
     c.lpw = lpw;
     c.block = dim3(lpw * tpl);
     c.shmem = (size_t)lpw * line_bytes;
-    if (c.shmem > 65536) raise(KW_ERR_UNSUPPORTED, "line of length " + std::to_string(n) + " exceeds 64 KiB of LDS");
+    if (c.shmem > lds_max_)
+      raise(KW_ERR_UNSUPPORTED, "line of length " + std::to_string(n) + " exceeds the LDS of a workgroup");
     if (contig) {
       c.grid = dim3((unsigned)((pairs + lpw - 1) / lpw));
     } else {
@@ -785,8 +793,8 @@ class Sim final : public SimBase {
     c.n = (int)n;
     const int tpl = plan_tpl((int)n);
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
-    int lpw = 256 / tpl;
-    while (lpw > 1 && (size_t)lpw * line_bytes > 65536) lpw /= 2;
+    int lpw = std::max(256, std::min(512, 16 * tpl)) / tpl;   // as for strided passes
+    while (lpw > 1 && (size_t)lpw * line_bytes > lds_max_) lpw /= 2;
     lpw = tuned_lpw((int)sizeof(T), cls, (int)n, lpw);
     c.req = lpw;
     while (!batch_mode_ && lpw > 1 && lpw / 2 >= NH_) lpw /= 2;
@@ -1240,7 +1248,7 @@ class Sim final : public SimBase {
     const int tpl = plan_tpl((int)n);
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
     int lpw = 256 / tpl;
-    while (lpw > 1 && (size_t)lpw * line_bytes > 65536) lpw /= 2;
+    while (lpw > 1 && (size_t)lpw * line_bytes > lds_max_) lpw /= 2;
     while (lpw > 1 && lpw / 2 >= lines) lpw /= 2;
     c.lpw = lpw;
     c.block = dim3(lpw * tpl);
@@ -1385,6 +1393,7 @@ class Sim final : public SimBase {
   bool het_c_ = false, nonlinear_ = false, has_p0_ = false, absorbing_ = false, no_record_ = false;
   bool axisym_ = false;
   DevBuf inv_r_;
+  size_t lds_max_ = 65536;   // LDS a workgroup may allocate on this device
   double c_ref_ = 0, tau_ = 0, y_ = 0, anp_ = 0, d_[3] = {1, 1, 1};
   std::vector<long long> psrc_idx_h_;
   std::vector<double> psrc_sig_h_, psrc_fac_h_;

@@ -522,7 +522,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
 }
 
 template <typename T, int N, bool CONTIG, bool TWO = false>
-__global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
+__global__ void __launch_bounds__(512) k_dir_pass(PassArgs<T> a) {
   dir_pass_body<T, N, CONTIG, TWO>(a, blockIdx.x, a.t);
 }
 
@@ -530,7 +530,7 @@ __global__ void __launch_bounds__(256) k_dir_pass(PassArgs<T> a) {
 // same line length; wg_sim maps a workgroup to its simulation, wg_first is
 // each simulation's first workgroup, *t_dev the current 1-based step.
 template <typename T, int N, bool CONTIG, bool TWO = false>
-__global__ void __launch_bounds__(256) k_dir_pass_batch(const PassArgs<T>* __restrict__ args,
+__global__ void __launch_bounds__(512) k_dir_pass_batch(const PassArgs<T>* __restrict__ args,
                                                         const int* __restrict__ wg_sim,
                                                         const int* __restrict__ wg_first, const int* t_dev) {
   const int s = wg_sim[blockIdx.x];
@@ -681,12 +681,12 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
 }
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(256) k_spec_pass(SpecArgs<T> a) {
+__global__ void __launch_bounds__(512) k_spec_pass(SpecArgs<T> a) {
   spec_pass_body<T, N, MODE>(a, blockIdx.x);
 }
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(256) k_spec_pass_batch(const SpecArgs<T>* __restrict__ args,
+__global__ void __launch_bounds__(512) k_spec_pass_batch(const SpecArgs<T>* __restrict__ args,
                                                          const int* __restrict__ wg_sim,
                                                          const int* __restrict__ wg_first) {
   const int s = wg_sim[blockIdx.x];
@@ -1062,7 +1062,7 @@ struct AdjArgs {
 };
 
 template <typename T, int N, bool CONTIG>
-__global__ void __launch_bounds__(256) k_adj_pass(AdjArgs<T> a) {
+__global__ void __launch_bounds__(512) k_adj_pass(AdjArgs<T> a) {
   constexpr int V = CONTIG ? 1 : 0;
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
