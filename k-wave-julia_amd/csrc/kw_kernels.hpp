@@ -186,28 +186,82 @@ struct LineCtx {
     return f16 ? off + (T)__half2float(at32(reinterpret_cast<const __half*>(f) + wgb, i))
                : at32(reinterpret_cast<const T*>(f) + wgb, i);
   }
+  // The storage test is hoisted out of the element loops so each loop is one
+  // straight run of independent loads.
   __device__ __forceinline__ void load_map(const void* f, Cx<T> (&v)[E], bool f16, T off, T fill) const {
     if (vec) {
+      if (f16) {
+        load_half_pairs(f, v, off, fill, 0);
+      } else {
 #pragma unroll
-      for (int q = 0; q < E; ++q) v[q] = va ? map_pair(f, ia(q), f16, off) : Cx<T>{fill, fill};
+        for (int q = 0; q < E; ++q) v[q] = va ? map_pair(f, ia(q), false, off) : Cx<T>{fill, fill};
+      }
       return;
+    }
+    if (f16) {
+      load_half_lines(f, v, off, fill, 0);
+    } else {
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        v[q].x = va ? map_at(f, ia(q), false, off) : fill;
+        v[q].y = vb ? map_at(f, ib(q), false, off) : fill;
+      }
+    }
+  }
+  // FP16 map pairs (strided lines): raw 32-bit loads under one lane test, then conversion
+  __device__ __forceinline__ void load_half_pairs(const void* f, Cx<T> (&v)[E], T off, T fill, int next) const {
+    const unsigned* __restrict__ h2 =
+        reinterpret_cast<const unsigned*>(reinterpret_cast<const unsigned short*>(f) + wgb);
+    unsigned r[E] = {};
+    if (va) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) r[q] = at32(h2, (next ? oa + nx(q) : ia(q)) >> 1);
     }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      v[q].x = va ? map_at(f, ia(q), f16, off) : fill;
-      v[q].y = vb ? map_at(f, ib(q), f16, off) : fill;
+      const __half lo = __ushort_as_half((unsigned short)(r[q] & 0xFFFFu));
+      const __half hi = __ushort_as_half((unsigned short)(r[q] >> 16));
+      v[q] = va ? Cx<T>{off + (T)__half2float(lo), off + (T)__half2float(hi)} : Cx<T>{fill, fill};
+    }
+  }
+  // FP16 map elements of lines a and b: all raw 16-bit loads of a line are
+  // issued under one lane test, then converted (a conversion inside each
+  // conditional load made every load wait for the previous one).
+  __device__ __forceinline__ void load_half_lines(const void* f, Cx<T> (&v)[E], T off, T fill, int next) const {
+    const unsigned short* __restrict__ h = reinterpret_cast<const unsigned short*>(f) + wgb;
+    unsigned short ra[E] = {}, rb[E] = {};
+    if (va) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) ra[q] = at32(h, next ? oa + nx(q) : ia(q));
+    }
+    if (vb) {
+#pragma unroll
+      for (int q = 0; q < E; ++q) rb[q] = at32(h, next ? ob + nx(q) : ib(q));
+    }
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      v[q].x = va ? off + (T)__half2float(__ushort_as_half(ra[q])) : fill;
+      v[q].y = vb ? off + (T)__half2float(__ushort_as_half(rb[q])) : fill;
     }
   }
   __device__ __forceinline__ void load_map_next(const void* f, Cx<T> (&v)[E], bool f16, T off, T fill) const {
     if (vec) {
+      if (f16) {
+        load_half_pairs(f, v, off, fill, 1);
+      } else {
 #pragma unroll
-      for (int q = 0; q < E; ++q) v[q] = va ? map_pair(f, oa + nx(q), f16, off) : Cx<T>{fill, fill};
+        for (int q = 0; q < E; ++q) v[q] = va ? map_pair(f, oa + nx(q), false, off) : Cx<T>{fill, fill};
+      }
       return;
     }
+    if (f16) {
+      load_half_lines(f, v, off, fill, 1);
+    } else {
 #pragma unroll
-    for (int q = 0; q < E; ++q) {
-      v[q].x = va ? map_at(f, oa + nx(q), f16, off) : fill;
-      v[q].y = vb ? map_at(f, ob + nx(q), f16, off) : fill;
+      for (int q = 0; q < E; ++q) {
+        v[q].x = va ? map_at(f, oa + nx(q), false, off) : fill;
+        v[q].y = vb ? map_at(f, ob + nx(q), false, off) : fill;
+      }
     }
   }
 };
