@@ -5,7 +5,8 @@ sensor}.jl and solver/first_order.jl) over the C ABI of libkwave_hip.so
 (include/kwave_hip.h).  The time loop runs only on the GPU; if the shared
 library or the GPU is missing, the solver raises.
 """
-from .model import (AbstractKWaveGrid, Additive, AdditiveNoCorrection, Dirichlet, KWaveGrid, KWaveGrid1D,
+from .model import (AbstractKWaveGrid, Additive, AdditiveNoCorrection, Dirichlet, ElasticMedium, ElasticSource,
+                    KWaveGrid, KWaveGrid1D,
                     KWaveGrid2D, KWaveGrid3D, KWaveMedium, KWaveSensor, KWaveSource, SimulationOutput,
                     SourceMode, VALID_RECORD_FIELDS, has_p0, has_pressure_source, has_velocity_source,
                     is_homogeneous, is_lossless, is_nonlinear, is_time_reversal, make_time,
@@ -21,7 +22,7 @@ from . import tuning
 from ._lib import KWaveHIPError, LIB_PATH
 
 __all__ = [
-    "AbstractKWaveGrid", "KWaveGrid", "KWaveGrid1D", "KWaveGrid2D", "KWaveGrid3D", "make_time",
+    "AbstractKWaveGrid", "ElasticMedium", "ElasticSource", "KWaveGrid", "KWaveGrid1D", "KWaveGrid2D", "KWaveGrid3D", "make_time",
     "KWaveMedium", "KWaveSource", "KWaveSensor", "SimulationOutput", "SourceMode", "Dirichlet",
     "Additive", "AdditiveNoCorrection", "VALID_RECORD_FIELDS", "validate_record_fields",
     "is_homogeneous", "is_lossless", "is_nonlinear", "is_time_reversal", "has_p0",

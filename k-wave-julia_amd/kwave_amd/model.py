@@ -210,6 +210,39 @@ class KWaveSource:
         self.u_mode = SourceMode(u_mode)
 
 
+class ElasticMedium:
+    """ElasticMedium(; sound_speed_compression, sound_speed_shear, density=1000,
+    alpha_coeff_compression, alpha_coeff_shear, alpha_power) (elastic.jl:20-48)."""
+
+    def __init__(self, *, sound_speed_compression, sound_speed_shear, density=1000.0,
+                 alpha_coeff_compression=None, alpha_coeff_shear=None, alpha_power=None):
+        self.sound_speed_compression = _f64(sound_speed_compression)
+        self.sound_speed_shear = _f64(sound_speed_shear)
+        self.density = _f64(density)
+        self.alpha_coeff_compression = _f64(alpha_coeff_compression)
+        self.alpha_coeff_shear = _f64(alpha_coeff_shear)
+        self.alpha_power = None if alpha_power is None else float(alpha_power)
+
+
+class ElasticSource:
+    """ElasticSource(; s_mask, sxx, syy, szz, sxy, sxz, syz, s_mode=Additive, u_mask,
+    ux, uy, uz, u_mode=Additive, p0) (elastic.jl:57-103)."""
+
+    def __init__(self, *, s_mask=None, sxx=None, syy=None, szz=None, sxy=None, sxz=None, syz=None,
+                 s_mode: SourceMode = Additive, u_mask=None, ux=None, uy=None, uz=None,
+                 u_mode: SourceMode = Additive, p0=None):
+        f = lambda x: None if x is None else np.asarray(x, dtype=np.float64)
+        b = lambda x: None if x is None else np.asarray(x, dtype=bool)
+        self.s_mask = b(s_mask)
+        self.sxx, self.syy, self.szz = f(sxx), f(syy), f(szz)
+        self.sxy, self.sxz, self.syz = f(sxy), f(sxz), f(syz)
+        self.s_mode = SourceMode(s_mode)
+        self.u_mask = b(u_mask)
+        self.ux, self.uy, self.uz = f(ux), f(uy), f(uz)
+        self.u_mode = SourceMode(u_mode)
+        self.p0 = f(p0)
+
+
 def has_p0(s: KWaveSource) -> bool:
     return s.p0 is not None
 

@@ -642,6 +642,150 @@ def kspace_first_order_as(kgrid, medium, source, sensor, **kwargs) -> Dict[str, 
     return kspace_first_order(kgrid, medium, source, sensor, axisymmetric=True, **kwargs)
 
 
+def pstd_elastic(kgrid, medium, source, sensor, *, pml_size=None, pml_alpha=2.0, data_cast=np.float64,
+                 progress_callback=None, **_ignored) -> Dict[str, np.ndarray]:
+    """Restatement of pstd_elastic_2d / pstd_elastic_3d (elastic.jl:133-368, 392-588).
+
+    Stress-velocity PSTD without k-space correction: v_a <- Psg_a (Psg_a v_a + dt/rho_sg,a sum_b
+    D_b+ s_ab); normal stresses s_aa <- P_a (P_a s_aa + dt (lam div v + 2 mu D_a- v_a)); shear
+    s_ab <- P_a P_b (s_ab + dt mu (D_b- v_a + D_a- v_b)).  2-D with scalar Lame parameters adds the
+    Kelvin-Voigt factors (1 + chi_p), (1 + chi_s) (elastic.jl:303-316); 3-D has none.  p = -tr(s)/ndim.
+    """
+    g = grid_from(kgrid)
+    nd = g.ndim
+    if nd not in (2, 3):
+        raise ValueError("pstd_elastic needs a 2-D or 3-D grid")
+    shape = g.N
+    T = np.dtype(data_cast).type
+    if g.Nt == 0 or g.dt == 0:
+        raise ValueError("Time array not set. Call make_time!(kgrid, sound_speed) before running.")
+    record = [str(r) for r in (sensor.record or [])]
+    for f in record:
+        if f not in VALID_RECORD_FIELDS:
+            raise ValueError(f"Invalid sensor record field: :{f}")
+    dt, Nt = g.dt, g.Nt
+    if pml_size is None:
+        pml_size = 20 if nd == 2 else 10                                      # elastic.jl:137, 396
+    psz = _norm_tuple(pml_size, nd, int)
+    pal = _norm_tuple(pml_alpha, nd, float)
+
+    def bs(v, ax):
+        s = [1] * nd
+        s[ax] = -1
+        return v.reshape(s)
+
+    P = [bs(get_pml(g.N[ax], g.d[ax], psz[ax], pal[ax], dt), ax) for ax in range(nd)]
+    Psg = [bs(get_pml(g.N[ax], g.d[ax], psz[ax], pal[ax], dt, staggered=True), ax) for ax in range(nd)]
+    cp = _as_param(medium.sound_speed_compression)
+    cs = _as_param(medium.sound_speed_shear)
+    rho0 = _as_param(medium.density)
+    lam = rho0 * (cp ** 2 - 2 * cs ** 2)                                      # elastic.jl:186-190
+    mu = rho0 * cs ** 2
+    scalar_lame = np.ndim(lam) == 0
+    chi_p = chi_s = 0.0
+    y = getattr(medium, "alpha_power", None)
+    if nd == 2 and y is not None:                                             # elastic.jl:193-210
+        if getattr(medium, "alpha_coeff_compression", None) is not None:
+            c_ref = cp if np.ndim(cp) == 0 else float(np.mean(cp))
+            chi_p = 2 * db2neper(float(medium.alpha_coeff_compression), y) * c_ref ** (y - 1)
+        if getattr(medium, "alpha_coeff_shear", None) is not None:
+            cs_ref = cs if np.ndim(cs) == 0 else float(np.mean(cs))
+            chi_s = 2 * db2neper(float(medium.alpha_coeff_shear), y) * cs_ref ** (y - 1)
+    if not scalar_lame:
+        chi_p = chi_s = 0.0                      # the array branch carries no viscous terms
+    rho_sg = [rho0 if np.ndim(rho0) == 0 else stagger_density(rho0, ax) for ax in range(nd)]
+    Dp = lambda f, ax: spectral_gradient(f, g.k_vecs[ax], g.shifts[ax][0], ax)
+    Dm = lambda f, ax: spectral_gradient(f, g.k_vecs[ax], g.shifts[ax][1], ax)
+
+    v = [np.zeros(shape, dtype=T) for _ in range(nd)]
+    sn = [np.zeros(shape, dtype=T) for _ in range(nd)]                        # sxx, syy, (szz)
+    pairs = [(0, 1)] if nd == 2 else [(0, 1), (0, 2), (1, 2)]                 # sxy, (sxz, syz)
+    ss = {pr: np.zeros(shape, dtype=T) for pr in pairs}
+    p0 = getattr(source, "p0", None)
+    if p0 is not None:
+        for ax in range(nd):
+            sn[ax][...] = -np.asarray(p0, dtype=np.float64)
+
+    mask = getattr(sensor, "mask", None)
+    data: Dict[str, np.ndarray] = {}
+    mask_idx = np.zeros(0, dtype=np.int64)
+    if mask is not None:
+        m = np.asarray(mask)
+        n_sensor = _findall(m).size if m.dtype == bool else m.shape[1]
+        if m.dtype == bool:
+            mask_idx = _findall(m)
+        for f in record:
+            if f in ("p", "ux", "uy", "uz"):
+                data[f] = np.zeros((n_sensor, Nt))
+            elif f in ("p_max", "u_max", "I_max"):
+                data[f] = np.full(n_sensor, -np.inf)
+            elif f == "p_min":
+                data[f] = np.full(n_sensor, np.inf)
+            else:
+                data[f] = np.zeros(n_sensor)
+
+    def inject(arr, idx, sig, t, dirichlet):
+        s = np.asarray(sig, dtype=np.float64)
+        s = s[:, None] if s.ndim == 1 else s
+        col = min(t, s.shape[1]) - 1
+        flat = arr.ravel(order="F").copy()
+        if dirichlet:
+            flat[idx] = s[:, col]
+        else:
+            np.add.at(flat, idx, s[:, col])
+        arr[...] = flat.reshape(arr.shape, order="F")
+
+    u_mask = getattr(source, "u_mask", None)
+    s_mask = getattr(source, "s_mask", None)
+    u_idx = _findall(u_mask) if u_mask is not None else None
+    s_idx = _findall(s_mask) if s_mask is not None else None
+    u_dir = int(getattr(source, "u_mode", ADDITIVE)) == DIRICHLET
+    s_dir = int(getattr(source, "s_mode", ADDITIVE)) == DIRICHLET
+
+    def s_of(a_, b_):
+        if a_ == b_:
+            return sn[a_]
+        return ss[(min(a_, b_), max(a_, b_))]
+
+    for t in range(1, Nt + 1):
+        for ax in range(nd):                                                  # momentum
+            div = sum(Dp(s_of(ax, b), b) for b in range(nd))
+            v[ax][...] = Psg[ax] * (Psg[ax] * v[ax] + dt / rho_sg[ax] * div)
+        if u_idx is not None:
+            for ax, nm in enumerate(("ux", "uy", "uz")[:nd]):
+                sig = getattr(source, nm, None)
+                if sig is not None:
+                    inject(v[ax], u_idx, sig, t, u_dir)
+        dv = [[Dm(v[ax], b) for b in range(nd)] for ax in range(nd)]          # dv[a][b] = d v_a / d x_b
+        div_v = sum(dv[ax][ax] for ax in range(nd))
+        for ax in range(nd):
+            inner = lam * div_v + 2 * mu * dv[ax][ax]
+            sn[ax][...] = P[ax] * (P[ax] * sn[ax] + dt * (1 + chi_p) * inner)
+        for (a_, b_) in pairs:
+            ss[(a_, b_)][...] = P[a_] * P[b_] * (ss[(a_, b_)] + dt * (1 + chi_s) * mu * (dv[a_][b_] + dv[b_][a_]))
+        if s_idx is not None:
+            names = ("sxx", "syy", "szz")[:nd]
+            for ax, nm in enumerate(names):
+                sig = getattr(source, nm, None)
+                if sig is not None:
+                    inject(sn[ax], s_idx, sig, t, s_dir)
+            for (a_, b_) in pairs:
+                nm = "s" + "xyz"[a_] + "xyz"[b_]
+                sig = getattr(source, nm, None)
+                if sig is not None:
+                    inject(ss[(a_, b_)], s_idx, sig, t, s_dir)
+        p = -sum(sn) / nd
+        if mask_idx.size:
+            _record(data, record, p, v, mask_idx, t, Nt)
+        if progress_callback is not None:
+            progress_callback(t, Nt, p)
+    if "p_rms" in data:
+        data["p_rms"] = np.sqrt(data["p_rms"] / Nt)
+    if "u_rms" in data:
+        data["u_rms"] = np.sqrt(data["u_rms"] / Nt)
+    return data
+
+
 def _record(data, record, p, u, idx, t, Nt):
     pf = p.ravel(order="F")
     uf = [a.ravel(order="F") for a in u]
