@@ -247,6 +247,66 @@ int32_t kw_batch_get_sensor(kw_batch* batch, int32_t sim, uint32_t field, double
 int32_t kw_batch_get_field(kw_batch* batch, int32_t sim, int32_t which, double* out);
 int32_t kw_batch_destroy(kw_batch* batch);
 
+/*
+ * Elastic stress-velocity PSTD: replaces the time loops of
+ * pstd_elastic_2d (KWave.jl/src/solver/elastic.jl:133-368) and
+ * pstd_elastic_3d (elastic.jl:386-588).  Velocities v_a and stresses s_ab
+ * on the staggered grid, no k-space correction; pressure p = -tr(s)/ndim
+ * is what the sensor records.  Axis lengths as for kw_sim_desc
+ * (kw_fft_length_supported); Cartesian sensor masks are preallocated and
+ * never filled, as in the reference.
+ */
+typedef struct kw_elastic kw_elastic;
+enum kw_elastic_field {
+  KW_EL_VX = 0, KW_EL_VY, KW_EL_VZ, KW_EL_SXX, KW_EL_SYY, KW_EL_SZZ, KW_EL_SXY, KW_EL_SXZ, KW_EL_SYZ
+};
+typedef struct kw_elastic_desc {
+  int32_t ndim;          /* 2 or 3 */
+  int64_t N[3];
+  double d[3];
+  double dt;
+  int64_t Nt;
+  int32_t precision;     /* 4 or 8 */
+  int32_t pml_size[3];   /* defaults 20 (2-D) / 10 (3-D) are the caller's (elastic.jl:138, 391) */
+  double pml_alpha[3];
+  const double* cp;      /* sound_speed_compression map or NULL (then cp_scalar) */
+  double cp_scalar;
+  const double* cs;      /* sound_speed_shear map or NULL (then cs_scalar) */
+  double cs_scalar;
+  const double* rho0;    /* density map or NULL (then rho0_scalar) */
+  double rho0_scalar;
+  double chi_p;          /* Kelvin-Voigt factors (elastic.jl:193-210); 0 where the reference */
+  double chi_s;          /*   applies none (3-D, or any heterogeneous Lame parameter) */
+  const double* p0;      /* normal stresses start at -p0 (elastic.jl:217-222, 458-464) or NULL */
+  /* velocity source at u_mask (elastic.jl:248-275): ux, uy, uz are n_usrc x ncols */
+  int64_t n_usrc;
+  const int64_t* usrc_index;      /* findall(u_mask), 0-based */
+  const double* usrc_signal[3];
+  int64_t usrc_ncols[3];
+  int32_t u_mode;                 /* kw_source_mode (Dirichlet or additive) */
+  /* stress source at s_mask (elastic.jl:312-345): sxx syy szz sxy sxz syz */
+  int64_t n_ssrc;
+  const int64_t* ssrc_index;      /* findall(s_mask), 0-based */
+  const double* ssrc_signal[6];
+  int64_t ssrc_ncols[6];
+  int32_t s_mode;
+  /* sensor (record_sensor_data!, first_order_steps.jl) */
+  int64_t n_sensor;
+  const int64_t* sensor_index;    /* findall(mask), 0-based; NULL with n_sensor>0 = Cartesian */
+  uint32_t record;                /* kw_record bits */
+} kw_elastic_desc;
+int32_t kw_elastic_create(kw_ctx* ctx, const kw_elastic_desc* desc, kw_elastic** out);
+int32_t kw_elastic_run(kw_elastic* sim, int64_t n_steps);
+int32_t kw_elastic_synchronize(kw_elastic* sim);
+int32_t kw_elastic_steps_done(const kw_elastic* sim, int64_t* steps);
+/* which: kw_elastic_field; the full grid, column-major */
+int32_t kw_elastic_get_field(kw_elastic* sim, int32_t which, double* out);
+/* p(t) of the pressure -tr(s)/ndim over the whole grid (progress callbacks) */
+int32_t kw_elastic_get_pressure(kw_elastic* sim, double* out);
+int32_t kw_elastic_sensor_shape(kw_elastic* sim, uint32_t field, int64_t* rows, int64_t* cols);
+int32_t kw_elastic_get_sensor(kw_elastic* sim, uint32_t field, double* out);
+int32_t kw_elastic_destroy(kw_elastic* sim);
+
 #ifdef __cplusplus
 }
 #endif

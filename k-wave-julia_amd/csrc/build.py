@@ -30,7 +30,10 @@ SOURCES = [
     "kw_launch_batch_f64.hip",
     "kw_launch_adj.hip",
     "kw_launch_misc.hip",
+    "kw_launch_el_f32.hip",
+    "kw_launch_el_f64.hip",
     "kw_api.hip",
+    "kw_elastic.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

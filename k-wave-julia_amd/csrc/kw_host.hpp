@@ -1,0 +1,173 @@
+// Host-side helpers shared by the solver translation units: error
+// reporting, device buffers, FP64 grid tables (wavenumbers, PML profiles,
+// twiddles, staggered derivative multipliers) and line-pass launch shapes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/kwave_hip.h"
+#include "kw_launch.hpp"
+
+namespace kwh {
+
+using kw::Cx;
+
+inline thread_local std::string g_err;
+
+struct KwError {
+  int32_t code;
+  std::string msg;
+};
+
+[[noreturn]] inline void raise(int32_t code, const std::string& m) { throw KwError{code, m}; }
+
+#define HIPC(x)                                                                         \
+  do {                                                                                  \
+    hipError_t e_ = (x);                                                                \
+    if (e_ != hipSuccess) raise(KW_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Device memory
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void alloc(size_t b) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = b;
+    if (b == 0) return;
+    hipError_t e = hipMalloc(&p, b);
+    if (e != hipSuccess) {
+      p = nullptr;
+      raise(KW_ERR_OOM, "hipMalloc of " + std::to_string(b) + " bytes failed: " + hipGetErrorString(e));
+    }
+  }
+  template <typename U>
+  U* as() const {
+    return reinterpret_cast<U*>(p);
+  }
+};
+
+template <typename U>
+void upload(DevBuf& b, const std::vector<U>& h) {
+  b.alloc(h.size() * sizeof(U));
+  if (!h.empty()) HIPC(hipMemcpy(b.p, h.data(), h.size() * sizeof(U), hipMemcpyHostToDevice));
+}
+
+// ---------------------------------------------------------------------------
+// Host tables (FP64, cast once to the kernel precision)
+// ---------------------------------------------------------------------------
+// grid.jl:120-130
+inline std::vector<double> wavenumbers(long long n, double d) {
+  std::vector<double> k(n);
+  const double dk = 2.0 * M_PI / ((double)n * d);
+  for (long long i = 0; i < n; ++i) {
+    long long m;
+    if (n % 2 == 0)
+      m = (i < n / 2) ? i : i - n;
+    else
+      m = (i <= (n - 1) / 2) ? i : i - n;
+    k[i] = (double)m * dk;
+  }
+  return k;
+}
+
+// pml.jl:25-55
+inline std::vector<double> get_pml(long long n, int size, double alpha, double dt, bool staggered) {
+  std::vector<double> p(n, 1.0);
+  for (int i = 1; i <= size; ++i) {
+    const double pos = staggered ? ((double)(size - i) + 0.5) / size : (double)(size - i + 1) / size;
+    const double v = std::exp(-(alpha * std::pow(pos, 4)) * dt);
+    p[i - 1] = v;
+    p[n - i] = v;
+  }
+  return p;
+}
+
+// exp(-2 pi i m / n) with exact symmetric reduction of the angle
+inline void twiddles(long long n, std::vector<double>& re, std::vector<double>& im) {
+  re.resize(n);
+  im.resize(n);
+  for (long long m = 0; m < n; ++m) {
+    // angle = -2 pi m/n; evaluate via the smallest equivalent argument
+    long long mm = m;
+    double s = -1.0;
+    if (2 * mm > n) {
+      mm = n - mm;
+      s = 1.0;
+    }
+    const double a = 2.0 * M_PI * (double)mm / (double)n;
+    re[m] = std::cos(a);
+    im[m] = s * std::sin(a);
+  }
+}
+
+template <typename T>
+std::vector<Cx<T>> to_cx(const std::vector<double>& re, const std::vector<double>& im) {
+  std::vector<Cx<T>> o(re.size());
+  for (size_t i = 0; i < re.size(); ++i) o[i] = Cx<T>{(T)re[i], (T)im[i]};
+  return o;
+}
+template <typename T>
+std::vector<T> to_t(const std::vector<double>& v) {
+  std::vector<T> o(v.size());
+  for (size_t i = 0; i < v.size(); ++i) o[i] = (T)v[i];
+  return o;
+}
+
+struct AxisTables {
+  std::vector<double> twr, twi, dpr, dpi, dmr, dmi, pml, pml_sg;
+};
+
+// Staggered derivative multipliers i k exp(+-i k h/2) (grid.jl:149-153,
+// fft_utils.jl:146-175) with the inverse 1/N folded in.  The reference's
+// irfft keeps only the real part of the DC and Nyquist bins of the product;
+// taking the real part of the multiplier there reproduces that exactly for
+// the pair-of-real-lines transform.
+inline AxisTables axis_tables(long long n, double h, int pml_size, double pml_alpha, double dt) {
+  AxisTables A;
+  twiddles(n, A.twr, A.twi);
+  std::vector<double> k = wavenumbers(n, h);
+  A.dpr.resize(n);
+  A.dpi.resize(n);
+  A.dmr.resize(n);
+  A.dmi.resize(n);
+  for (long long i = 0; i < n; ++i) {
+    const double c = std::cos(k[i] * h / 2), s = std::sin(k[i] * h / 2);
+    // i k (c + i s) = -k s + i k c ; i k (c - i s) = k s + i k c
+    double pr = -k[i] * s, pi = k[i] * c;
+    double mr = k[i] * s, mi = k[i] * c;
+    if (i == 0 || (n % 2 == 0 && i == n / 2)) {
+      pi = 0.0;
+      mi = 0.0;
+    }
+    A.dpr[i] = pr / (double)n;
+    A.dpi[i] = pi / (double)n;
+    A.dmr[i] = mr / (double)n;
+    A.dmi[i] = mi / (double)n;
+  }
+  A.pml = get_pml(n, pml_size, pml_alpha, dt, false);
+  A.pml_sg = get_pml(n, pml_size, pml_alpha, dt, true);
+  return A;
+}
+
+struct PassCfg {
+  int n = 0;        // line length
+  int lpw = 1;      // lines (pairs) per workgroup
+  int req = 1;      // lines per workgroup asked for (before shrinking to small grids)
+  dim3 grid, block;
+  size_t shmem = 0;
+};
+
+}  // namespace kwh

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "kw_elastic.hpp"
 #include "kw_kernels.hpp"
 
 // Supported line-FFT lengths (kw_fft.hpp Plan<N>).
@@ -66,5 +67,14 @@ hipError_t launch_r2c(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const T* 
                       const Cx<T>* tw);
 template <typename T>
 hipError_t launch_adj_small(int which, hipStream_t s, const void* const* p, const double* dargs, long long n, int ndim);
+
+// Elastic stress-velocity passes and the small per-step kernels around them.
+template <typename T>
+hipError_t launch_el(int n, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const ElArgs<T>& a);
+template <typename T>
+hipError_t launch_el_sensor_p(hipStream_t s, T* p, const T* s0, const T* s1, const T* s2, const long long* idx, int n,
+                              int ndim);
+template <typename T>
+hipError_t launch_el_inject(hipStream_t s, T* f, const long long* idx, const T* vals, int n, int dirichlet);
 
 }  // namespace kw

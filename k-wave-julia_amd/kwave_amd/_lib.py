@@ -36,7 +36,11 @@ EXPORTS = (
     "kw_sim_profile", "kw_sim_grad_c0", "kw_sim_destroy", "kw_autotune", "kw_tune_export", "kw_tune_import",
     "kw_tune_clear", "kw_batch_create", "kw_batch_run", "kw_batch_synchronize",
     "kw_batch_sensor_shape", "kw_batch_get_sensor", "kw_batch_get_field", "kw_batch_destroy",
+    "kw_elastic_create", "kw_elastic_run", "kw_elastic_synchronize", "kw_elastic_steps_done",
+    "kw_elastic_get_field", "kw_elastic_get_pressure", "kw_elastic_sensor_shape", "kw_elastic_get_sensor",
+    "kw_elastic_destroy",
 )
+EL_FIELD = {"vx": 0, "vy": 1, "vz": 2, "sxx": 3, "syy": 4, "szz": 5, "sxy": 6, "sxz": 7, "syz": 8}
 
 
 class KWaveHIPError(RuntimeError):
@@ -98,6 +102,42 @@ class SimDesc(C.Structure):
     ]
 
 
+class ElasticDesc(C.Structure):
+    """Mirror of kw_elastic_desc (include/kwave_hip.h)."""
+    _fields_ = [
+        ("ndim", C.c_int32),
+        ("N", C.c_int64 * 3),
+        ("d", C.c_double * 3),
+        ("dt", C.c_double),
+        ("Nt", C.c_int64),
+        ("precision", C.c_int32),
+        ("pml_size", C.c_int32 * 3),
+        ("pml_alpha", C.c_double * 3),
+        ("cp", C.POINTER(C.c_double)),
+        ("cp_scalar", C.c_double),
+        ("cs", C.POINTER(C.c_double)),
+        ("cs_scalar", C.c_double),
+        ("rho0", C.POINTER(C.c_double)),
+        ("rho0_scalar", C.c_double),
+        ("chi_p", C.c_double),
+        ("chi_s", C.c_double),
+        ("p0", C.POINTER(C.c_double)),
+        ("n_usrc", C.c_int64),
+        ("usrc_index", C.POINTER(C.c_int64)),
+        ("usrc_signal", C.POINTER(C.c_double) * 3),
+        ("usrc_ncols", C.c_int64 * 3),
+        ("u_mode", C.c_int32),
+        ("n_ssrc", C.c_int64),
+        ("ssrc_index", C.POINTER(C.c_int64)),
+        ("ssrc_signal", C.POINTER(C.c_double) * 6),
+        ("ssrc_ncols", C.c_int64 * 6),
+        ("s_mode", C.c_int32),
+        ("n_sensor", C.c_int64),
+        ("sensor_index", C.POINTER(C.c_int64)),
+        ("record", C.c_uint32),
+    ]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -144,6 +184,15 @@ def load():
             "kw_batch_get_sensor": ([P, C.c_int32, C.c_uint32, C.POINTER(C.c_double)], C.c_int32),
             "kw_batch_get_field": ([P, C.c_int32, C.c_int32, C.POINTER(C.c_double)], C.c_int32),
             "kw_batch_destroy": ([P], C.c_int32),
+            "kw_elastic_create": ([P, C.POINTER(ElasticDesc), C.POINTER(P)], C.c_int32),
+            "kw_elastic_run": ([P, C.c_int64], C.c_int32),
+            "kw_elastic_synchronize": ([P], C.c_int32),
+            "kw_elastic_steps_done": ([P, C.POINTER(C.c_int64)], C.c_int32),
+            "kw_elastic_get_field": ([P, C.c_int32, C.POINTER(C.c_double)], C.c_int32),
+            "kw_elastic_get_pressure": ([P, C.POINTER(C.c_double)], C.c_int32),
+            "kw_elastic_sensor_shape": ([P, C.c_uint32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int32),
+            "kw_elastic_get_sensor": ([P, C.c_uint32, C.POINTER(C.c_double)], C.c_int32),
+            "kw_elastic_destroy": ([P], C.c_int32),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)

@@ -1,0 +1,155 @@
+"""GPU parity of pstd_elastic_2d / pstd_elastic_3d (elastic.jl:133-588) against
+the oracle on the same seeded inputs: FP64 kernels within 1e-10 rel-L2, FP32
+kernels within 1e-4 of the FP64 oracle."""
+import copy
+
+import numpy as np
+import pytest
+
+import kwave_amd as K
+from oracle import kwave_oracle as O
+
+from cases import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(shape, d=1e-4):
+    args = []
+    for n in shape:
+        args += [n, d]
+    return K.KWaveGrid(*args)
+
+
+def _mask(rng, shape, n):
+    m = np.zeros(int(np.prod(shape)), dtype=bool)
+    m[rng.choice(m.size, size=n, replace=False)] = True
+    return m.reshape(shape, order="F")
+
+
+def elastic_case(name, shape, seed=3, nt=50):
+    rng = np.random.default_rng(seed)
+    nd = len(shape)
+    kg = _grid(shape)
+    het = "het" in name
+    cp = 2500.0 + 600.0 * rng.random(shape) if het else 2800.0
+    cs = 1100.0 + 300.0 * rng.random(shape) if het else 1300.0
+    rho = 1600.0 + 400.0 * rng.random(shape) if het else 1800.0
+    K.make_time(kg, cp)
+    kg.Nt = nt
+    kg.t_array = np.arange(nt) * kg.dt
+    mkw = dict(sound_speed_compression=cp, sound_speed_shear=cs, density=rho)
+    if "kv" in name:
+        # coefficients large enough that (1 + chi) ~ 1.1 shows in the data
+        mkw.update(alpha_coeff_compression=3e9, alpha_coeff_shear=4e9, alpha_power=1.3)
+    med = K.ElasticMedium(**mkw)
+    skw = {}
+    if "p0" in name:
+        p0 = np.zeros(shape)
+        c = tuple(n // 2 for n in shape)
+        sl = tuple(slice(ci - 2, ci + 3) for ci in c)
+        p0[sl] = 1e5 * rng.random(p0[sl].shape)
+        skw["p0"] = p0
+    burst = K.tone_burst(1 / kg.dt, 1.5e6, 2)
+    sig = np.zeros(nt)
+    sig[: min(nt, burst.size)] = burst[:nt]
+    if "ssrc" in name:
+        nsrc = 6
+        skw["s_mask"] = _mask(rng, shape, nsrc)
+        skw["s_mode"] = K.Dirichlet if "sdir" in name else K.Additive
+        skw["sxx"] = 1e5 * np.outer(rng.random(nsrc), sig)
+        skw["sxy"] = 5e4 * np.outer(rng.random(nsrc), sig)
+        if nd == 3:
+            skw["szz"] = 1e5 * np.outer(rng.random(nsrc), sig)
+            skw["syz"] = 4e4 * np.outer(rng.random(nsrc), sig[: nt // 2])      # fewer columns: last one held
+        else:
+            skw["syy"] = 8e4 * rng.random(nsrc)                                  # one column for every step
+    if "usrc" in name:
+        nsrc = 5
+        skw["u_mask"] = _mask(rng, shape, nsrc)
+        skw["u_mode"] = K.Dirichlet if "udir" in name else K.Additive
+        skw["ux"] = 0.05 * np.outer(rng.random(nsrc), sig)
+        if nd == 3:
+            skw["uz"] = 0.03 * np.outer(rng.random(nsrc), sig)
+        else:
+            skw["uy"] = 0.03 * np.outer(rng.random(nsrc), sig)
+    src = K.ElasticSource(**skw)
+    record = ["p", "ux", "uy"] + (["uz"] if nd == 3 else [])
+    if "stats" in name:
+        record = ["p", "p_max", "p_min", "p_rms", "p_final", "u_max", "u_rms", "u_final", "I_avg", "I_max", "ux"]
+    sen = K.KWaveSensor(mask=_mask(rng, shape, 40), record=record)
+    pml = 6 if nd == 2 else 4
+    return kg, med, src, sen, dict(pml_size=pml, pml_alpha=2.0)
+
+
+CASES_2D = [
+    ("p0", (64, 48)),
+    ("het_p0_stats", (48, 64)),
+    ("kv_ssrc", (64, 64)),
+    ("het_ssrc_sdir_usrc", (40, 96)),
+    ("p0_usrc_udir", (80, 32)),
+]
+CASES_3D = [
+    ("p0", (32, 24, 16)),
+    ("het_ssrc_usrc", (24, 32, 20)),
+    ("het_p0_stats", (20, 16, 24)),
+]
+
+
+def _check(out, ref, tol, label):
+    for k in ref.keys():
+        e = rel_l2(out[k], ref[k]) if np.any(ref[k]) else float(np.abs(np.asarray(out[k])).max())
+        print(f"    {label} {k:8s} rel-L2 {e:.3e}", flush=True)
+        assert e <= tol, (k, e)
+
+
+@pytest.mark.parametrize("name,shape", CASES_2D + CASES_3D, ids=[f"{c[0]}_{len(c[1])}d" for c in CASES_2D + CASES_3D])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_elastic_matches_oracle(name, shape, dtype):
+    print(f"\n[case] elastic {name} {shape} {dtype.__name__}", flush=True)
+    kg, med, src, sen, kw = elastic_case(name, shape, seed=11 + len(name), nt=50)
+    fn = K.pstd_elastic_2d if len(shape) == 2 else K.pstd_elastic_3d
+    out = fn(copy.deepcopy(kg), med, src, sen, data_cast=dtype, **kw)
+    ref = O.pstd_elastic(copy.deepcopy(kg), med, src, sen, **kw)
+    assert set(out.keys()) == set(ref.keys())
+    _check(out, ref, 1e-10 if dtype == np.float64 else 1e-4, dtype.__name__)
+
+
+def test_elastic_fields_and_progress():
+    print("\n[case] elastic fields + progress callback 2d f64", flush=True)
+    kg, med, src, sen, kw = elastic_case("het_ssrc_usrc", (48, 40), nt=20)
+    seen = []
+    out = K.pstd_elastic_2d(copy.deepcopy(kg), med, src, sen, data_cast=np.float64,
+                            progress_callback=lambda t, nt, p: seen.append((t, p.copy())), **kw)
+    ref_seen = []
+    ref = O.pstd_elastic(copy.deepcopy(kg), med, src, sen,
+                         progress_callback=lambda t, nt, p: ref_seen.append((t, np.array(p))), **kw)
+    assert [t for t, _ in seen] == list(range(1, 21))
+    for (t, p), (_, q) in zip(seen, ref_seen):
+        assert rel_l2(p, q) <= 1e-10, t
+    _check(out, ref, 1e-10, "f64")
+
+
+def test_elastic_direct_handle_fields():
+    print("\n[case] elastic direct handle 3d f64 (all nine fields)", flush=True)
+    from kwave_amd.elastic import ElasticSimulation, build_elastic_desc
+    kg, med, src, sen, kw = elastic_case("het_p0_stats", (16, 20, 24), nt=12)
+    d, keep, info = build_elastic_desc(kg, med, src, sen, data_cast=np.float64, **kw)
+    with ElasticSimulation(d, keep) as sim:
+        sim.run(12)
+        assert sim.steps_done() == 12
+        p = sim.pressure()
+        tr = sim.field("sxx") + sim.field("syy") + sim.field("szz")
+        assert np.allclose(p, -tr / 3, rtol=0, atol=1e-12 * np.abs(tr).max())
+        for f in ("vx", "vy", "vz", "sxy", "sxz", "syz"):
+            assert np.isfinite(sim.field(f)).all() and np.abs(sim.field(f)).max() > 0
+
+
+def test_elastic_cartesian_mask_never_filled():
+    print("\n[case] elastic Cartesian sensor 2d f32", flush=True)
+    kg, med, src, _, kw = elastic_case("p0", (32, 32), nt=10)
+    sen = K.KWaveSensor(mask=np.zeros((2, 7)), record=["p", "p_max", "p_min"])
+    out = K.pstd_elastic_2d(copy.deepcopy(kg), med, src, sen, data_cast=np.float32, **kw)
+    ref = O.pstd_elastic(copy.deepcopy(kg), med, src, sen, **kw)
+    for k in ("p", "p_max", "p_min"):
+        assert np.array_equal(out[k], ref[k]), k
