@@ -269,6 +269,87 @@ def bench_c4(a):
     print(json.dumps(out))
 
 
+def cpu_baseline_elastic(n: int = 128, n_steps_max: int = 12, budget_s: float = 15.0):
+    """Oracle elastic restatement (FP32, scipy.fft threads) on an n^3 cut of the workload."""
+    import numpy as np
+    from oracle import kwave_oracle as O
+    from kwave_amd.configs import config_elastic
+
+    workers = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    workers = min(workers, 16)
+    O.set_fft_workers(workers)
+    kg, med, src, sen, kw = config_elastic(N=n, Nt=200)
+    stamps = []
+
+    class Stop(Exception):
+        pass
+
+    def cb(t, Nt, p):
+        stamps.append(time.perf_counter())
+        if len(stamps) >= 2 and (stamps[-1] - stamps[0] > budget_s or len(stamps) > n_steps_max):
+            raise Stop()
+
+    try:
+        O.pstd_elastic(kg, med, src, sen, data_cast=np.float32, progress_callback=cb, **kw)
+    except Stop:
+        pass
+    k = len(stamps) - 1
+    dt = stamps[-1] - stamps[0]
+    return {"value": n ** 3 * k / dt, "unit": "voxel-updates/s", "cores": workers, "kind": "port",
+            "sample": f"oracle FP32 elastic restatement on the {n}^3 cut of the workload: 1 warm-up step + {k} "
+                      f"timed steps ({dt:.1f} s; scipy.fft with {workers} threads)"}
+
+
+def bench_elastic(a):
+    """Elastic stress-velocity PSTD (pstd_elastic_3d): 3-D two-layer 256^3 FP32 workload."""
+    import numpy as np
+    import kwave_amd as K
+    from kwave_amd.configs import config_elastic, elastic_pass_bytes
+    from kwave_amd.elastic import ElasticSimulation, build_elastic_desc
+
+    N = a.N
+    kg, med, src, sen, kw = config_elastic(N=N, Nt=a.warmup + a.steps + a.profile_steps)
+    desc, keep, _ = build_elastic_desc(kg, med, src, sen, data_cast=np.float32, **kw)
+    sim = ElasticSimulation(desc, keep)
+    del keep
+    sim.run(a.warmup)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    sim.run(a.steps)
+    sim.synchronize()
+    t2 = time.perf_counter()
+    prof = sim.profile(a.profile_steps) if a.profile_steps > 0 else {}
+    pm = sim.sensor("p_max")
+    assert np.all(np.isfinite(pm)) and pm.max() > 0
+    sim.close()
+    vox = N ** 3
+    ms_step = (t2 - t1) * 1e3 / a.steps
+    table = elastic_pass_bytes(4)
+    passes, step_bytes = {}, 0.0
+    for name, (b, launches) in table.items():
+        step_bytes += b
+        if name in prof:
+            ms = prof[name]["ms"]
+            per_launch = b / launches * vox
+            passes[name] = {"ms": round(ms, 4), "launches_per_step": launches,
+                            "GB_s": round(per_launch / (ms * 1e-3) / 1e9, 1)}
+    dom = max(passes, key=lambda k: passes[k]["ms"] * passes[k]["launches_per_step"]) if passes else None
+    out = {
+        "metric": "voxel-updates/s (elastic, FP32)", "value": vox * a.steps / (t2 - t1), "unit": "voxel-updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"3D two-layer elastic {N}^3 dx=0.1mm PML 10 (fluid cp1540 / solid cp3000 cs1500), "
+                               "p0 ball, z-plane sensor p + p_max", "parallelism": "single GPU"},
+        "step_roofline": {"bound": "hbm", "bytes_per_voxel_step": step_bytes,
+                          "achieved": round(step_bytes * vox / (ms_step * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(step_bytes * vox / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)},
+        "dominant_pass": dom, "passes": passes,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline_elastic()
+    print(json.dumps(out))
+
+
 def copy_probe():
     """A plain elementwise copy of known size (1 GiB read + 1 GiB written per launch), for
     calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE against algorithmic bytes."""
@@ -291,7 +372,7 @@ def main():
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 accuracy run")
-    ap.add_argument("--config", choices=["c2", "batch", "adjoint", "c4"], default="c2")
+    ap.add_argument("--config", choices=["c2", "batch", "adjoint", "c4", "elastic"], default="c2")
     ap.add_argument("--firings", type=int, default=8)
     ap.add_argument("--fd-pixels", type=int, default=32)
     ap.add_argument("--c4-maps", choices=["fp16", "fp32"], default="fp16")
@@ -314,6 +395,10 @@ def main():
         if a.steps == 1200:
             a.steps = 200
         return bench_c4(a)
+    if a.config == "elastic":
+        if a.steps == 1200:
+            a.steps = 200
+        return bench_elastic(a)
 
     import numpy as np
     import kwave_amd as K

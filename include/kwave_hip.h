@@ -305,6 +305,9 @@ int32_t kw_elastic_get_field(kw_elastic* sim, int32_t which, double* out);
 int32_t kw_elastic_get_pressure(kw_elastic* sim, double* out);
 int32_t kw_elastic_sensor_shape(kw_elastic* sim, uint32_t field, int64_t* rows, int64_t* cols);
 int32_t kw_elastic_get_sensor(kw_elastic* sim, uint32_t field, double* out);
+/* Timing support as kw_sim_profile: per-pass-class mean device time (ms) over n_steps. */
+int32_t kw_elastic_profile(kw_elastic* sim, int64_t n_steps, int32_t max_classes, int32_t* n_classes,
+                           const char** names, double* ms, int64_t* launches);
 int32_t kw_elastic_destroy(kw_elastic* sim);
 
 #ifdef __cplusplus

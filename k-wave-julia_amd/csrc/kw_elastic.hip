@@ -71,6 +71,37 @@ class ElasticSim {
     for (long long i = 0; i < nvox_; ++i) out[i] = -out[i] / (double)nd_;
   }
 
+  // per-kernel-class mean device time over n steps (HIP events on the stream)
+  void profile(long long n, int maxc, int* nc, const char** names, double* ms, long long* launches) {
+    HIPC(hipSetDevice(dev_));
+    std::vector<ProfRec> ev;
+    ev.reserve(64);
+    prof_ = &ev;
+    for (long long i = 0; i < n && t_ < Nt_; ++i) step();
+    prof_ = nullptr;
+    HIPC(hipStreamSynchronize(stream_));
+    double tot[kNumClasses] = {};
+    long long cnt[kNumClasses] = {};
+    for (auto& r : ev) {
+      float e = 0.f;
+      HIPC(hipEventElapsedTime(&e, r.a, r.b));
+      tot[r.cls] += e;
+      cnt[r.cls] += 1;
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    int k = 0;
+    for (int c = 0; c < kNumClasses && k < maxc; ++c) {
+      if (cnt[c] == 0) continue;
+      names[k] = kClassNames[c];
+      ms[k] = tot[c] / (double)cnt[c];
+      launches[k] = cnt[c];
+      ++k;
+    }
+    *nc = k;
+    if (t_ == Nt_ && !finalized_) finalize();
+  }
+
   void sensor_shape(uint32_t f, long long* r, long long* c) {
     if (!(rec_ & f)) raise(KW_ERR_INVALID, "field not recorded");
     *r = nsens_;
@@ -375,9 +406,34 @@ class ElasticSim {
     const int k = lo == 0 ? hi - 1 : 2;   // (0,1) -> 0, (0,2) -> 1, (1,2) -> 2
     return ss_[k].template as<T>();
   }
+  // kernel classes: mode x line axis, then the small kernels
+  static constexpr int kNumClasses = 14;
+  static constexpr const char* kClassNames[kNumClasses] = {
+      "k_el_deriv_x", "k_el_deriv_y", "k_el_deriv_z", "k_el_vel_x",  "k_el_vel_y",   "k_el_vel_z",   "k_el_normal_x",
+      "k_el_normal_y", "k_el_normal_z", "k_el_shear_x", "k_el_shear_y", "k_el_shear_z", "k_el_inject", "k_el_record"};
+  enum { CLS_INJECT = 12, CLS_RECORD = 13 };
+  struct ProfRec {
+    int cls;
+    hipEvent_t a, b;
+  };
+  template <typename F>
+  void timed(int cls, F&& f) {
+    if (!prof_) {
+      f();
+      return;
+    }
+    ProfRec r;
+    r.cls = cls;
+    HIPC(hipEventCreate(&r.a));
+    HIPC(hipEventCreate(&r.b));
+    HIPC(hipEventRecord(r.a, stream_));
+    f();
+    HIPC(hipEventRecord(r.b, stream_));
+    prof_->push_back(r);
+  }
   void launch(int axis, const ElArgs<T>& A) {
     const PassCfg& c = cfg_[axis];
-    HIPC(launch_el<T>(c.n, axis == 0, c.grid, c.block, c.shmem, stream_, A));
+    timed(3 * A.mode + axis, [&] { HIPC(launch_el<T>(c.n, axis == 0, c.grid, c.block, c.shmem, stream_, A)); });
   }
   // out (+)= D_axis f
   void deriv(int axis, const T* f, const Cx<T>* D, T* out, bool accumulate) {
@@ -414,8 +470,10 @@ class ElasticSim {
     for (int a = 0; a < nd_; ++a) {
       if (!uval_[a].p) continue;
       const long long col = std::min<long long>(t, u_nc_[a]) - 1;
-      HIPC(launch_el_inject<T>(stream_, v_[a].template as<T>(), uidx_.as<long long>(),
-                               uval_[a].template as<T>() + col * u_ns_, (int)u_ns_, u_dir_ ? 1 : 0));
+      timed(CLS_INJECT, [&] {
+        HIPC(launch_el_inject<T>(stream_, v_[a].template as<T>(), uidx_.as<long long>(),
+                                 uval_[a].template as<T>() + col * u_ns_, (int)u_ns_, u_dir_ ? 1 : 0));
+      });
     }
     // normal stresses (elastic.jl:288-316, 540-547): D_a- v_a
     deriv(0, v_[0].template as<T>(), Dm_[0].template as<Cx<T>>(), s0, false);
@@ -449,16 +507,20 @@ class ElasticSim {
       if (!sval_[k].p) continue;
       T* f = k < 3 ? sn_[k].template as<T>() : ss_[k - 3].template as<T>();
       const long long col = std::min<long long>(t, s_nc_[k]) - 1;
-      HIPC(launch_el_inject<T>(stream_, f, sidx_src_.as<long long>(), sval_[k].template as<T>() + col * s_ns_,
-                               (int)s_ns_, s_dir_ ? 1 : 0));
+      timed(CLS_INJECT, [&] {
+        HIPC(launch_el_inject<T>(stream_, f, sidx_src_.as<long long>(), sval_[k].template as<T>() + col * s_ns_,
+                                 (int)s_ns_, s_dir_ ? 1 : 0));
+      });
     }
     // record p = -tr(s)/ndim and the velocities at the sensor points
     if (nsens_ > 0 && !cartesian_) {
-      HIPC(launch_el_sensor_p<T>(stream_, p_.as<T>(), sn_[0].template as<T>(), sn_[1].template as<T>(),
-                                 nd_ == 3 ? sn_[2].template as<T>() : nullptr, sens_idx_.as<long long>(),
-                                 (int)nsens_, nd_));
       ra_.t = t;
-      HIPC(launch_record<T>((unsigned)((nsens_ + 255) / 256), stream_, ra_));
+      timed(CLS_RECORD, [&] {
+        HIPC(launch_el_sensor_p<T>(stream_, p_.as<T>(), sn_[0].template as<T>(), sn_[1].template as<T>(),
+                                   nd_ == 3 ? sn_[2].template as<T>() : nullptr, sens_idx_.as<long long>(),
+                                   (int)nsens_, nd_));
+        HIPC(launch_record<T>((unsigned)((nsens_ + 255) / 256), stream_, ra_));
+      });
     }
     t_ = t;
   }
@@ -494,7 +556,11 @@ class ElasticSim {
   uint32_t rec_ = 0;
   DevBuf sens_idx_, p_ts_, u_ts_[3], p_max_, p_min_, p_rms_, p_final_, u_max_, u_rms_, u_final_, I_avg_, I_max_;
   RecArgs<T> ra_{};
+  std::vector<ProfRec>* prof_ = nullptr;
 };
+
+template <typename T>
+constexpr const char* ElasticSim<T>::kClassNames[];
 
 struct ElBase {
   virtual ~ElBase() = default;
@@ -505,6 +571,7 @@ struct ElBase {
   virtual void get_pressure(double* out) = 0;
   virtual void sensor_shape(uint32_t f, long long* r, long long* c) = 0;
   virtual void get_sensor(uint32_t f, double* out) = 0;
+  virtual void profile(long long n, int maxc, int* nc, const char** names, double* ms, long long* launches) = 0;
 };
 template <typename T>
 struct ElImpl final : ElBase {
@@ -516,6 +583,9 @@ struct ElImpl final : ElBase {
   void get_pressure(double* out) override { s.get_pressure(out); }
   void sensor_shape(uint32_t f, long long* r, long long* c) override { s.sensor_shape(f, r, c); }
   void get_sensor(uint32_t f, double* out) override { s.get_sensor(f, out); }
+  void profile(long long n, int maxc, int* nc, const char** names, double* ms, long long* launches) override {
+    s.profile(n, maxc, nc, names, ms, launches);
+  }
   ElasticSim<T> s;
 };
 
@@ -617,6 +687,20 @@ int32_t kw_elastic_get_sensor(kw_elastic* sim, uint32_t field, double* out) {
   KW_GUARD_BEGIN
   if (!sim || !out) raise(KW_ERR_INVALID, "null argument");
   sim->impl->get_sensor(field, out);
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_elastic_profile(kw_elastic* sim, int64_t n_steps, int32_t max_classes, int32_t* n_classes,
+                           const char** names, double* ms, int64_t* launches) {
+  KW_GUARD_BEGIN
+  if (!sim || !n_classes || !names || !ms || !launches) raise(KW_ERR_INVALID, "null argument");
+  if (n_steps < 0) raise(KW_ERR_INVALID, "negative step count");
+  int nc = 0;
+  std::vector<long long> l(std::max(0, (int)max_classes));
+  sim->impl->profile(n_steps, max_classes, &nc, names, ms, l.data());
+  for (int i = 0; i < nc; ++i) launches[i] = l[i];
+  *n_classes = nc;
   return KW_OK;
   KW_GUARD_END
 }

@@ -238,3 +238,52 @@ def config5(N: int = 512, Nt: int = 1600, n_elements: int = 128, radius: float =
         firings.append((KWaveSource(p_mask=pm, p=sig), KWaveSensor(mask=ring, record=["p"])))
     c_start = np.full((N, N), 1500.0)
     return kg, c_true, c_start, firings, dict(pml_size=20, pml_alpha=2.0, smooth_p0=False)
+
+
+def config_elastic(N: int = 256, Nt: int = 600, pml: int = 10, seed: int = 42):
+    """3-D two-layer elastic workload (N^3, dx = 0.1 mm, PML 10) for pstd_elastic_3d.
+
+    z < N/2: fluid-like layer (cp 1540 m/s, cs 0, rho 1000); z >= N/2: solid
+    (cp 3000, cs 1500, rho 1900); N(0, 10 m/s) seeded noise on cp.  Initial
+    pressure: a ball of radius 6 cells, 1 MPa, centred at (N/2, N/2, N/4);
+    sensor: the plane z = N/2 - 8 recording p and p_max.
+    Returns (kgrid, medium, source, sensor, kwargs).
+    """
+    from .builders import make_ball
+    from .model import ElasticMedium, ElasticSource
+
+    kg = KWaveGrid(N, 1e-4, N, 1e-4, N, 1e-4)
+    solid = (np.arange(N) >= N // 2)[None, None, :]
+    noise = np.random.default_rng(seed).standard_normal((N, N, N), dtype=np.float32)
+    cp = np.where(solid, 3000.0, 1540.0) + 10.0 * noise
+    del noise
+    cs = np.broadcast_to(np.where(solid, 1500.0, 0.0), (N, N, N)).astype(np.float64)
+    rho = np.broadcast_to(np.where(solid, 1900.0, 1000.0), (N, N, N)).astype(np.float64)
+    make_time(kg, cp)
+    kg.Nt = Nt
+    kg.t_array = np.arange(Nt) * kg.dt
+    med = ElasticMedium(sound_speed_compression=cp, sound_speed_shear=cs, density=rho)
+    src = ElasticSource(p0=1e6 * make_ball(N, N, N, N // 2, N // 2, N // 4, 6).astype(np.float64))
+    mask = np.zeros((N, N, N), dtype=bool)
+    mask[:, :, N // 2 - 8] = True
+    sen = KWaveSensor(mask=mask, record=["p", "p_max"])
+    return kg, med, src, sen, {"pml_size": pml, "pml_alpha": 2.0}
+
+
+def elastic_pass_bytes(precision: int = 4):
+    """Bytes per voxel each class of elastic pass must move on the 3-D heterogeneous
+    workload, as (bytes per launch summed over a step, launches per step).
+
+    A derivative pass reads its input and writes (2 fields) or accumulates into
+    (3 fields) a scratch field; a velocity pass reads the stress, the partial
+    sum, v and rho and writes v (5); the normal-stress pass reads v_z, two
+    derivatives, lambda, mu and three stresses and writes three (11); a shear
+    pass reads v, the partial sum, mu and the stress and writes it (5).
+    """
+    s = precision
+    return {
+        "k_el_deriv_x": (3 * 2 * s, 3), "k_el_deriv_y": ((3 * 2 + 3) * s, 4),
+        "k_el_deriv_z": ((2 * 3 + 2 * 2) * s, 4), "k_el_vel_x": (5 * s, 1), "k_el_vel_y": (5 * s, 1),
+        "k_el_vel_z": (5 * s, 1), "k_el_normal_z": (11 * s, 1), "k_el_shear_x": (2 * 5 * s, 2),
+        "k_el_shear_y": (5 * s, 1),
+    }

@@ -193,6 +193,16 @@ class ElasticSimulation:
         _lib.check(self._lib.kw_elastic_get_pressure(self._h, out.ctypes.data_as(_DP)))
         return out.reshape(self.shape, order="F")
 
+    def profile(self, n_steps: int) -> Dict[str, Dict[str, float]]:
+        """Run n_steps with HIP events around every launch; per-pass-class mean ms."""
+        m = 32
+        names = (C.c_char_p * m)()
+        ms = (C.c_double * m)()
+        cnt = (C.c_int64 * m)()
+        nc = C.c_int32()
+        _lib.check(self._lib.kw_elastic_profile(self._h, int(n_steps), m, C.byref(nc), names, ms, cnt))
+        return {names[i].decode(): {"ms": ms[i], "launches": int(cnt[i])} for i in range(nc.value)}
+
     def sensor(self, name: str) -> np.ndarray:
         bit = _lib.REC_BITS[name]
         r, c = C.c_int64(), C.c_int64()
