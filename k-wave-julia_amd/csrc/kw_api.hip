@@ -516,8 +516,9 @@ class Sim final : public SimBase {
       if (N_[a] < 1) raise(KW_ERR_INVALID, "grid size must be positive");
       if (!size_supported(N_[a]))
         raise(KW_ERR_UNSUPPORTED, "axis length " + std::to_string(N_[a]) +
-                                      " has no line-FFT plan (supported: 8,10,12,16,20,24,32,40,48,64,80,96,"
-                                      "128,160,192,256,320,384,512,640,768,1024,2048,4096)");
+                                      " has no line-FFT plan (supported: 8,9,10,12,15,16,20,24,25,27,32,36,"
+                                      "40,45,48,64,72,75,80,81,96,100,125,128,135,144,160,192,200,225,243,256,"
+                                      "320,384,400,512,640,768,1024,2048,4096)");
       if (d.pml_size[a] < 0) raise(KW_ERR_INVALID, "pml_size must be >= 0");
       if (2LL * d.pml_size[a] > N_[a] + 0 && d.pml_size[a] > N_[a])
         raise(KW_ERR_INVALID, "pml_size larger than the grid");

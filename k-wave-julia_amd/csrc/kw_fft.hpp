@@ -267,6 +267,24 @@ KW_PLAN(768, 64, 12, 4, 4, 4)
 KW_PLAN(1024, 64, 16, 16, 4)
 KW_PLAN(2048, 128, 16, 16, 8)
 KW_PLAN(4096, 256, 16, 16, 16)
+// odd and other composite lengths (grids need not be powers of two)
+KW_PLAN(9, 1, 9)
+KW_PLAN(15, 1, 15)
+KW_PLAN(25, 5, 5, 5)
+KW_PLAN(27, 3, 9, 3)
+KW_PLAN(36, 3, 12, 3)
+KW_PLAN(45, 3, 15, 3)
+KW_PLAN(72, 6, 12, 6)
+KW_PLAN(75, 5, 15, 5)
+KW_PLAN(81, 9, 9, 9)
+KW_PLAN(100, 10, 10, 10)
+KW_PLAN(125, 25, 5, 5, 5)
+KW_PLAN(135, 9, 15, 3, 3)
+KW_PLAN(144, 12, 12, 12)
+KW_PLAN(200, 10, 20, 10)
+KW_PLAN(225, 15, 15, 15)
+KW_PLAN(243, 27, 9, 9, 3)
+KW_PLAN(400, 20, 20, 20)
 #undef KW_PLAN
 template <int N>
 struct Plan<N, 1> : Plan<N, 0> {};

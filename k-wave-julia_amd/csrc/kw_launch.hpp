@@ -8,8 +8,9 @@
 
 // Supported line-FFT lengths (kw_fft.hpp Plan<N>).
 #define KW_SIZES(X) \
-  X(8) X(10) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(80) X(96) X(128) X(160) X(192) X(256) X(320) \
-  X(384) X(512) X(640) X(768) X(1024) X(2048) X(4096)
+  X(8) X(9) X(10) X(12) X(15) X(16) X(20) X(24) X(25) X(27) X(32) X(36) X(40) X(45) X(48) X(64) X(72) X(75) \
+  X(80) X(81) X(96) X(100) X(125) X(128) X(135) X(144) X(160) X(192) X(200) X(225) X(243) X(256) X(320)  \
+  X(384) X(400) X(512) X(640) X(768) X(1024) X(2048) X(4096)
 
 namespace kw {
 

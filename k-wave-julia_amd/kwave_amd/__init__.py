@@ -19,12 +19,15 @@ from .solver import Simulation, build_desc, kspace_first_order, kspace_first_ord
 from .batch import Batch, kspace_first_order_batch
 from .adjoint import gradient_c0, misfit_rrule
 from .elastic import ElasticSimulation, pstd_elastic_2d, pstd_elastic_3d
+from .transducer import (KWaveTransducer, combine_transducer_sensor_data, get_transducer_binary_mask,
+                         get_transducer_source)
 from . import tuning
 from ._lib import KWaveHIPError, LIB_PATH
 
 __all__ = [
     "AbstractKWaveGrid", "ElasticMedium", "ElasticSource", "ElasticSimulation", "pstd_elastic_2d",
-    "pstd_elastic_3d", "KWaveGrid", "KWaveGrid1D", "KWaveGrid2D", "KWaveGrid3D", "make_time",
+    "pstd_elastic_3d", "KWaveTransducer", "get_transducer_binary_mask", "get_transducer_source",
+    "combine_transducer_sensor_data", "KWaveGrid", "KWaveGrid1D", "KWaveGrid2D", "KWaveGrid3D", "make_time",
     "KWaveMedium", "KWaveSource", "KWaveSensor", "SimulationOutput", "SourceMode", "Dirichlet",
     "Additive", "AdditiveNoCorrection", "VALID_RECORD_FIELDS", "validate_record_fields",
     "is_homogeneous", "is_lossless", "is_nonlinear", "is_time_reversal", "has_p0",

@@ -88,11 +88,13 @@ CASES_2D = [
     ("kv_ssrc", (64, 64)),
     ("het_ssrc_sdir_usrc", (40, 96)),
     ("p0_usrc_udir", (80, 32)),
+    ("het_ssrc_usrc", (45, 27)),
 ]
 CASES_3D = [
     ("p0", (32, 24, 16)),
     ("het_ssrc_usrc", (24, 32, 20)),
     ("het_p0_stats", (20, 16, 24)),
+    ("het_p0_usrc", (25, 15, 27)),
 ]
 
 

@@ -1,6 +1,7 @@
 """GPU runs driven by an off-grid transducer array (array.jl): one firing per
 element through the batch form, element signals recovered with
-combine_sensor_data, against the oracle on the same distributed inputs."""
+combine_sensor_data, against the oracle on the same distributed inputs; and
+a focused, steered, apodized linear-transducer firing (transducer.jl)."""
 import copy
 
 import numpy as np
@@ -45,3 +46,28 @@ def test_array_firings_batch_matches_oracle():
         assert rel_l2(per_elem, ref_elem) <= 1e-10
         # the firing element hears its own pulse first and loudest
         assert np.abs(per_elem[e]).max() > np.abs(per_elem[1 - e]).max()
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_transducer_firing_matches_oracle(dtype):
+    print(f"\n[case] linear transducer firing 3-D {dtype.__name__}", flush=True)
+    kg = K.KWaveGrid(32, 1e-4, 64, 1e-4, 32, 1e-4)
+    K.make_time(kg, 1540.0)
+    kg.Nt = 60
+    kg.t_array = np.arange(60) * kg.dt
+    med = K.KWaveMedium(sound_speed=1540.0, density=1000.0)
+    tr = K.KWaveTransducer(number_elements=8, element_width=3, element_spacing=1, element_length=6,
+                           position=(6, 8, 12), focus_distance=2e-3, steering_angle=0.1,
+                           transmit_apodization="hann", receive_apodization="hamming",
+                           input_signal=1e5 * K.tone_burst(1 / kg.dt, 2e6, 2))
+    mask, sig = K.get_transducer_source(tr, kg, med)
+    src = K.KWaveSource(p_mask=mask, p=sig)
+    sen = K.KWaveSensor(mask=mask, record=["p"])
+    kw = dict(pml_size=6, pml_alpha=2.0, smooth_p0=False)
+    out = K.kspace_first_order(copy.deepcopy(kg), med, src, sen, data_cast=dtype, **kw)
+    ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
+    tol = 1e-10 if dtype == np.float64 else 1e-4
+    assert rel_l2(out["p"], ref["p"]) <= tol
+    got = K.combine_transducer_sensor_data(tr, kg, out["p"])
+    want = K.combine_transducer_sensor_data(tr, kg, ref["p"])
+    assert got.shape == (8, 60) and rel_l2(got, want) <= tol

@@ -41,6 +41,12 @@ CASES = [
     ("stokes_p0_nonlin_psrc", (16, 32, 24)),
     ("hetc_hetr_usrc_urec_p0", (16, 16, 32)),
     ("pdir_psrc_irec", (24, 24, 24)),
+    # odd and non-power-of-two axes (non-paired strided loads, odd half spectra)
+    ("hetc_hetr_nonlin_psrc", (45, 27)),
+    ("stokes_p0_psrc", (25, 75)),
+    ("hetc_absorb_nonlin_psrc", (27, 15, 25)),
+    ("p0_stokes_psrc", (81,)),
+    ("hetc_hetr_usrc_urec_p0", (36, 100)),
 ]
 
 
@@ -69,7 +75,8 @@ def test_parity_vs_oracle(name, shape, dtype):
     _compare(out, ref, TOL[dtype], f"{name}/{dtype.__name__}")
 
 
-SIZES = [8, 10, 12, 16, 20, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192, 256, 320, 384, 512, 640, 768, 1024, 2048, 4096]
+SIZES = [8, 9, 10, 12, 15, 16, 20, 24, 25, 27, 32, 36, 40, 45, 48, 64, 72, 75, 80, 81, 96, 100, 125, 128, 135, 144,
+         160, 192, 200, 225, 243, 256, 320, 384, 400, 512, 640, 768, 1024, 2048, 4096]
 
 
 @pytest.mark.parametrize("n", SIZES)
@@ -82,7 +89,8 @@ def test_every_line_length_1d(n):
     _compare(out, ref, 1e-10, f"1d-{n}")
 
 
-@pytest.mark.parametrize("shape", [(8, 384), (768, 12), (12, 96, 8), (8, 8, 192)])
+@pytest.mark.parametrize("shape", [(8, 384), (768, 12), (12, 96, 8), (8, 8, 192), (9, 45), (27, 25, 15),
+                                   (75, 36, 9), (15, 81, 135)])
 def test_strided_line_lengths(shape):
     """Plans used on the strided (y/z) axes and the spectral passes (FP64)."""
     print(f"\n[case] strided {shape}", flush=True)

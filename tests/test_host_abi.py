@@ -31,9 +31,9 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     lib = _lib.load()
     assert lib.kw_abi_version() == 1
-    for n in (8, 24, 128, 256, 384, 512, 768, 1024, 4096):
+    for n in (8, 9, 15, 24, 25, 27, 45, 81, 100, 125, 128, 135, 225, 243, 256, 384, 400, 512, 768, 1024, 4096):
         assert lib.kw_fft_length_supported(n) == 1
-    for n in (7, 100, 1000, 8192):
+    for n in (7, 11, 101, 1000, 8192):
         assert lib.kw_fft_length_supported(n) == 0
 
 
