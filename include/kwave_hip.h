@@ -248,6 +248,27 @@ int32_t kw_batch_get_field(kw_batch* batch, int32_t sim, int32_t which, double* 
 int32_t kw_batch_destroy(kw_batch* batch);
 
 /*
+ * Operator-level entry points (parity tests of single operators).
+ * kw_test_axis_table: the per-axis tables the solvers build on the host, FP64:
+ *   KW_TAB_K       k vector (grid.jl:120-130), n values
+ *   KW_TAB_PML     get_pml (pml.jl:25-55), n values; KW_TAB_PML_SG the staggered profile
+ *   KW_TAB_DPLUS   i k exp(+i k d/2) (grid.jl:149-153), 2n values (re, im); KW_TAB_DMINUS with exp(-i k d/2);
+ *                  real part only at the even-n Nyquist bin (what the complex-to-real inverse keeps)
+ * kw_test_eos_table: the EOS multiplier of _compute_pressure! (first_order_steps.jl:245-289) on the
+ *   rfft grid (Nx/2+1, Ny, Nz), column-major, before the inverse transform's 1/N: which 0 =
+ *   c^2 kappa (1 + nabla1) (c^2 = 1 for a c0 map), which 1 = -c^2 kappa nabla2 (homogeneous :stokes).
+ * kw_test_spectral_gradient: spectral_gradient! (fft_utils.jl:126-182) of a real field along one axis
+ *   on the GPU: sign +1 / -1 = the staggered multipliers above, 0 = unstaggered i k (x axis, or an
+ *   odd-length axis: the N-D transform is line-separable there).
+ */
+enum kw_axis_table { KW_TAB_K = 0, KW_TAB_PML, KW_TAB_PML_SG, KW_TAB_DPLUS, KW_TAB_DMINUS };
+int32_t kw_test_axis_table(int64_t n, double d, int32_t pml_size, double pml_alpha, double dt, int32_t which,
+                           double* out);
+int32_t kw_test_eos_table(const kw_sim_desc* desc, int32_t which, double* out);
+int32_t kw_test_spectral_gradient(kw_ctx* ctx, int32_t precision, int32_t ndim, const int64_t* N, const double* d,
+                                  int32_t axis, int32_t sign, const double* f, double* out);
+
+/*
  * Elastic stress-velocity PSTD: replaces the time loops of
  * pstd_elastic_2d (KWave.jl/src/solver/elastic.jl:133-368) and
  * pstd_elastic_3d (elastic.jl:386-588).  Velocities v_a and stresses s_ab

@@ -34,6 +34,7 @@ SOURCES = [
     "kw_launch_el_f64.hip",
     "kw_api.hip",
     "kw_elastic.hip",
+    "kw_ops.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -660,49 +660,19 @@ class Sim final : public SimBase {
   // first_order_steps.jl:214-290) on the octant |kx| x |ky| x |kz| with the
   // inverse-transform 1/(Nx Ny Nz) folded in.
   void build_eos(const kw_sim_desc& d, bool het_c) {
-    const long long hx = N_[0] / 2 + 1, hy = N_[1] / 2 + 1, hz = N_[2] / 2 + 1;
-    std::vector<double> kx = wavenumbers(N_[0], d.d[0]);
-    std::vector<double> ky = nd_ > 1 ? wavenumbers(N_[1], d.d[1]) : std::vector<double>{0.0};
-    std::vector<double> kz = nd_ > 2 ? wavenumbers(N_[2], d.d[2]) : std::vector<double>{0.0};
+    const EosTables E = eos_tables(d, nd_, N_, het_c);
     const double c_ref = d.c_ref;
     const double inv_n = 1.0 / (double)nvox_;
-    // absorption prefactors (first_order.jl:13-30, conversion.jl:34)
-    bool absorbing = d.absorbing != 0 && d.alpha_mode != KW_NO_ABSORPTION;
-    double tau = 0, eta = 0, y = d.alpha_power;
-    if (absorbing) {
-      const double a_np = d.alpha_coeff * (1e-2 / (20.0 * std::log10(std::exp(1.0)))) *
-                          std::pow(2.0 * M_PI * 1e6, -y);
-      tau = -2.0 * a_np * std::pow(c_ref, y - 1.0);
-      eta = 2.0 * a_np * std::pow(c_ref, y - 1.0) * std::tan(M_PI * y / 2.0);
-    }
-    stokes_ = absorbing && !het_c && d.alpha_mode != KW_NO_DISPERSION;
+    stokes_ = E.stokes;
     c_ref_ = c_ref;
-    absorbing_ = absorbing;
-    tau_ = tau;
-    y_ = y;
-    anp_ = absorbing ? -tau / (2.0 * std::pow(c_ref, y - 1.0)) : 0.0;
+    absorbing_ = E.absorbing;
+    tau_ = E.tau;
+    y_ = d.alpha_power;
+    anp_ = E.absorbing ? -E.tau / (2.0 * std::pow(c_ref, y_ - 1.0)) : 0.0;
     for (int q = 0; q < 3; ++q) d_[q] = d.d[q];
-    const double c2 = het_c ? 1.0 : d.c0_scalar * d.c0_scalar;
-    std::vector<T> m1(hx * hy * hz), m2;
-    if (stokes_) m2.resize(m1.size());
-    for (long long iz = 0; iz < hz; ++iz)
-      for (long long iy = 0; iy < hy; ++iy)
-        for (long long ix = 0; ix < hx; ++ix) {
-          const double kk = std::sqrt(kx[ix] * kx[ix] + ky[iy] * ky[iy] + kz[iz] * kz[iz]);
-          const double arg = c_ref * kk * d.dt / 2;
-          const double kap = arg == 0.0 ? 1.0 : std::sin(arg) / arg;
-          double v1 = c2 * kap, v2 = 0.0;
-          if (absorbing) {
-            const double n1 = tau * std::pow(kk, y);
-            v1 = c2 * kap * (1.0 + n1);
-            if (stokes_) v2 = -c2 * kap * (eta * std::pow(kk, y - 1.0));
-          }
-          // layout [|ky|][|kz|][kx]: the multiplying pass (z lines in 3-D, y lines
-          // in 2-D) reads each table row as one compact block per outer index
-          const long long i = ix + hx * (iz + hz * iy);
-          m1[i] = (T)(v1 * inv_n);
-          if (stokes_) m2[i] = (T)(v2 * inv_n);
-        }
+    std::vector<T> m1(E.m1.size()), m2(E.m2.size());
+    for (size_t i = 0; i < m1.size(); ++i) m1[i] = (T)(E.m1[i] * inv_n);
+    for (size_t i = 0; i < m2.size(); ++i) m2[i] = (T)(E.m2[i] * inv_n);
     upload(M1_, m1);
     if (stokes_) upload(M2_, m2);
     // spectra

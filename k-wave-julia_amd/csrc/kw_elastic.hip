@@ -257,25 +257,8 @@ class ElasticSim {
     HIPC(hipStreamSynchronize(stream_));
   }
 
-  // lines per workgroup as for the acoustic passes: 256 threads on contiguous
-  // lines, >= 16 line pairs up to 512 threads on strided lines, within the LDS
   PassCfg line_cfg(long long n, bool contig, long long lines, long long outer) {
-    PassCfg c;
-    c.n = (int)n;
-    const int tpl = plan_tpl((int)n, contig ? 1 : 0);
-    const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
-    int lpw = (contig ? 256 : std::max(256, std::min(512, 16 * tpl))) / tpl;
-    while (lpw > 1 && (size_t)lpw * line_bytes > lds_max_) lpw /= 2;
-    const long long pairs = (lines + 1) / 2;
-    while (lpw > 1 && lpw / 2 >= pairs) lpw /= 2;
-    c.lpw = c.req = lpw;
-    c.block = dim3(lpw * tpl);
-    c.shmem = (size_t)lpw * line_bytes;
-    if (c.shmem > lds_max_)
-      raise(KW_ERR_UNSUPPORTED, "line of length " + std::to_string(n) + " exceeds the LDS of a workgroup");
-    const long long blocks = (pairs + lpw - 1) / lpw;
-    c.grid = dim3((unsigned)(contig ? blocks : blocks * std::max<long long>(outer, 1)));
-    return c;
+    return line_pass_cfg(n, contig, lines, outer, sizeof(Cx<T>), lds_max_);
   }
 
   ElArgs<T> base_args(int axis) {

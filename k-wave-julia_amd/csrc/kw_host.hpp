@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -169,5 +170,85 @@ struct PassCfg {
   dim3 grid, block;
   size_t shmem = 0;
 };
+
+// Lines per workgroup of a line pass: 256 threads on contiguous lines, >= 16
+// line pairs up to 512 threads on strided lines, halved until the lines fit
+// the workgroup's LDS and to the lines that exist.
+inline PassCfg line_pass_cfg(long long n, bool contig, long long lines, long long outer, size_t cx_bytes,
+                             size_t lds_max) {
+  PassCfg c;
+  c.n = (int)n;
+  const int tpl = kw::plan_tpl((int)n, contig ? 1 : 0);
+  const size_t line_bytes = (size_t)kw::plan_pitch((int)n) * cx_bytes;
+  int lpw = (contig ? 256 : std::max(256, std::min(512, 16 * tpl))) / tpl;
+  while (lpw > 1 && (size_t)lpw * line_bytes > lds_max) lpw /= 2;
+  const long long pairs = (lines + 1) / 2;
+  while (lpw > 1 && lpw / 2 >= pairs) lpw /= 2;
+  c.lpw = c.req = lpw;
+  c.block = dim3(lpw * tpl);
+  c.shmem = (size_t)lpw * line_bytes;
+  if (c.shmem > lds_max)
+    raise(KW_ERR_UNSUPPORTED, "line of length " + std::to_string(n) + " exceeds the LDS of a workgroup");
+  const long long blocks = (pairs + lpw - 1) / lpw;
+  c.grid = dim3((unsigned)(contig ? blocks : blocks * std::max<long long>(outer, 1)));
+  return c;
+}
+
+// EOS multipliers on the octant of |k| (first_order.jl:13-42, 361-375,
+// 581-595; first_order_steps.jl:245-289), FP64, without the 1/N of the
+// inverse transform: m1 = c^2 kappa (1 + tau |k|^y), m2 = -c^2 kappa eta |k|^(y-1)
+// (homogeneous :stokes only); c^2 = 1 for heterogeneous c0 (applied in real
+// space).  Layout [|ky|][|kz|][kx]: index ix + hx (iz + hz iy).
+struct EosTables {
+  long long hx = 1, hy = 1, hz = 1;
+  std::vector<double> m1, m2;
+  bool absorbing = false, stokes = false;
+  double tau = 0, eta = 0;
+};
+inline EosTables eos_tables(const kw_sim_desc& d, int nd, const long long N[3], bool het_c) {
+  EosTables E;
+  E.hx = N[0] / 2 + 1;
+  E.hy = N[1] / 2 + 1;
+  E.hz = N[2] / 2 + 1;
+  std::vector<double> kx = wavenumbers(N[0], d.d[0]);
+  std::vector<double> ky = nd > 1 ? wavenumbers(N[1], d.d[1]) : std::vector<double>{0.0};
+  std::vector<double> kz = nd > 2 ? wavenumbers(N[2], d.d[2]) : std::vector<double>{0.0};
+  const double c_ref = d.c_ref, y = d.alpha_power;
+  E.absorbing = d.absorbing != 0 && d.alpha_mode != KW_NO_ABSORPTION;
+  if (E.absorbing) {
+    const double a_np = d.alpha_coeff * (1e-2 / (20.0 * std::log10(std::exp(1.0)))) * std::pow(2.0 * M_PI * 1e6, -y);
+    E.tau = -2.0 * a_np * std::pow(c_ref, y - 1.0);
+    E.eta = 2.0 * a_np * std::pow(c_ref, y - 1.0) * std::tan(M_PI * y / 2.0);
+  }
+  E.stokes = E.absorbing && !het_c && d.alpha_mode != KW_NO_DISPERSION;
+  const double c2 = het_c ? 1.0 : d.c0_scalar * d.c0_scalar;
+  E.m1.resize(E.hx * E.hy * E.hz);
+  if (E.stokes) E.m2.resize(E.m1.size());
+  for (long long iz = 0; iz < E.hz; ++iz)
+    for (long long iy = 0; iy < E.hy; ++iy)
+      for (long long ix = 0; ix < E.hx; ++ix) {
+        const double kk = std::sqrt(kx[ix] * kx[ix] + ky[iy] * ky[iy] + kz[iz] * kz[iz]);
+        const double arg = c_ref * kk * d.dt / 2;
+        const double kap = arg == 0.0 ? 1.0 : std::sin(arg) / arg;
+        double v1 = c2 * kap, v2 = 0.0;
+        if (E.absorbing) {
+          v1 = c2 * kap * (1.0 + E.tau * std::pow(kk, y));
+          if (E.stokes) v2 = -c2 * kap * (E.eta * std::pow(kk, y - 1.0));
+        }
+        const long long i = ix + E.hx * (iz + E.hz * iy);
+        E.m1[i] = v1;
+        if (E.stokes) E.m2[i] = v2;
+      }
+  return E;
+}
+
+// Unstaggered derivative multiplier i k / n (real part kept at the Nyquist bin,
+// as a complex-to-real inverse transform does).
+inline void unshifted_derivative(long long n, double h, std::vector<double>& re, std::vector<double>& im) {
+  std::vector<double> k = wavenumbers(n, h);
+  re.assign(n, 0.0);
+  im.resize(n);
+  for (long long i = 0; i < n; ++i) im[i] = (n % 2 == 0 && i == n / 2) ? 0.0 : k[i] / (double)n;
+}
 
 }  // namespace kwh
