@@ -430,9 +430,11 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
     }
   }
 
-  // --- density update  rho = pml (pml rho - dt rho0 du); rho0 is re-read
-  // here (an L2 hit) rather than held in registers across the transforms
-  if (a.rho0 != nullptr) L.load_map(a.rho0, r0, a.maps_f16 != 0, a.rho0_off, (T)1);
+  // --- density update  rho = pml (pml rho - dt rho0 du).  Strided passes keep
+  // rho0 from the velocity update in registers (their register budget already
+  // allows two waves per SIMD, and a re-read misses L2 there); the contiguous
+  // x pass re-reads it (an L2 hit) to stay at three waves per SIMD.
+  if (CONTIG && a.rho0 != nullptr) L.load_map(a.rho0, r0, a.maps_f16 != 0, a.rho0_off, (T)1);
   {
     Cx<T> w[E];
     L.load(a.rho, w);
