@@ -106,7 +106,8 @@ __global__ void __launch_bounds__(512) k_el_pass(ElArgs<T> a) {
   geo.n_lines = a.n_lines;
   geo.x_blocks = a.x_blocks;
   int lpw;
-  const LineCtx<T, N, V> L = make_ctx<T, N, CONTIG>(geo, blockIdx.x, lpw);
+  const LineCtx<T, N, V> L =
+      make_ctx<T, N, CONTIG>(geo, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), lpw);
   Cx<T>* lds = lds_all + L.c * P;
   const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(a.tw, L.t);
 

@@ -25,6 +25,28 @@
 
 namespace kw {
 
+// Workgroup placement.  Workgroups are dispatched round-robin over the 8 XCDs
+// (hardware workgroup b runs on XCD b % 8), each with its own L2.  xcd_wg maps
+// b to a logical workgroup so that consecutive logical workgroups share an
+// XCD; it is a bijection on [0, n).
+__device__ __forceinline__ int xcd_wg(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7, s = b >> 3;
+  return x < r ? x * (q + 1) + s : r * (q + 1) + (x - r) * q + s;
+}
+// Strided and spectral passes whose workgroups cover less than 128 bytes of
+// each row even at 512 threads (lines of 768 points and more in FP32: 8 line
+// pairs) use it, so the two workgroups reading the halves of one 128-byte
+// segment run on one XCD and the second read hits its L2.  Wider rows keep
+// the plain order (decided at compile time: a runtime test cost the 256-point
+// passes about 1 %).
+template <typename T, int N>
+__device__ __forceinline__ int strided_wg(int b) {
+  if constexpr ((512 / Plan<N, 0>::TPL) * 2 * (int)sizeof(T) < 128)
+    return xcd_wg(b, (int)gridDim.x);
+  else
+    return b;
+}
+
 // ---------------------------------------------------------------------------
 // Arguments
 // ---------------------------------------------------------------------------
@@ -579,7 +601,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
 
 template <typename T, int N, bool CONTIG, bool TWO = false>
 __global__ void __launch_bounds__(512) k_dir_pass(PassArgs<T> a) {
-  dir_pass_body<T, N, CONTIG, TWO>(a, blockIdx.x, a.t);
+  dir_pass_body<T, N, CONTIG, TWO>(a, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), a.t);
 }
 
 // Batched form: one launch covers the workgroups of many simulations of the
@@ -589,8 +611,9 @@ template <typename T, int N, bool CONTIG, bool TWO = false>
 __global__ void __launch_bounds__(512) k_dir_pass_batch(const PassArgs<T>* __restrict__ args,
                                                         const int* __restrict__ wg_sim,
                                                         const int* __restrict__ wg_first, const int* t_dev) {
-  const int s = wg_sim[blockIdx.x];
-  dir_pass_body<T, N, CONTIG, TWO>(args[s], blockIdx.x - wg_first[s], *t_dev);
+  const int b = CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x);
+  const int s = wg_sim[b];
+  dir_pass_body<T, N, CONTIG, TWO>(args[s], b - wg_first[s], *t_dev);
 }
 
 // ---------------------------------------------------------------------------
@@ -738,15 +761,16 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
 
 template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(512) k_spec_pass(SpecArgs<T> a) {
-  spec_pass_body<T, N, MODE>(a, blockIdx.x);
+  spec_pass_body<T, N, MODE>(a, strided_wg<T, N>(blockIdx.x));
 }
 
 template <typename T, int N, int MODE>
 __global__ void __launch_bounds__(512) k_spec_pass_batch(const SpecArgs<T>* __restrict__ args,
                                                          const int* __restrict__ wg_sim,
                                                          const int* __restrict__ wg_first) {
-  const int s = wg_sim[blockIdx.x];
-  spec_pass_body<T, N, MODE>(args[s], blockIdx.x - wg_first[s]);
+  const int b = strided_wg<T, N>(blockIdx.x);
+  const int s = wg_sim[b];
+  spec_pass_body<T, N, MODE>(args[s], b - wg_first[s]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1131,7 +1155,8 @@ __global__ void __launch_bounds__(512) k_adj_pass(AdjArgs<T> a) {
   geo.n_lines = a.n_lines;
   geo.x_blocks = a.x_blocks;
   int lpw;
-  const LineCtx<T, N, V> L = make_ctx<T, N, CONTIG>(geo, blockIdx.x, lpw);
+  const LineCtx<T, N, V> L =
+      make_ctx<T, N, CONTIG>(geo, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), lpw);
   Cx<T>* lds = lds_all + L.c * P;
   const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(a.tw, L.t);
 
