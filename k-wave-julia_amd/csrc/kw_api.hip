@@ -22,7 +22,6 @@ using namespace kwh;
 
 namespace {
 
-
 // ---------------------------------------------------------------------------
 // Launch-shape tuning table: lines per workgroup for each (precision bytes,
 // pass class, line length), filled by kw_autotune's timed trials or by
@@ -59,7 +58,6 @@ struct SrcDev {
   DevBuf off, pos, row;
   bool any = false;
 };
-
 
 template <typename T>
 class Sim final : public SimBase {
@@ -1227,7 +1225,6 @@ class Sim final : public SimBase {
 template <typename T>
 constexpr const char* Sim<T>::kClassNames[];
 
-
 // ---------------------------------------------------------------------------
 // Batch of independent simulations: every pass is one launch per line length
 // across all simulations (workgroup -> simulation tables), the step index
@@ -1611,31 +1608,12 @@ void autotune_batch(int dev, const kw_sim_desc* d0, int n, int steps) {
 
 }  // namespace
 
-struct kw_ctx {
-  int device;
-};
 struct kw_sim {
   std::unique_ptr<SimBase> impl;
 };
 struct kw_batch {
   std::unique_ptr<BatchBase> impl;
 };
-
-#define KW_GUARD_BEGIN try {
-#define KW_GUARD_END                      \
-  }                                       \
-  catch (const KwError& e) {              \
-    g_err = e.msg;                        \
-    return e.code;                        \
-  }                                       \
-  catch (const std::exception& e) {       \
-    g_err = e.what();                     \
-    return KW_ERR_INVALID;                \
-  }                                       \
-  catch (...) {                           \
-    g_err = "unknown error";              \
-    return KW_ERR_INVALID;                \
-  }
 
 extern "C" {
 

@@ -574,28 +574,9 @@ struct ElImpl final : ElBase {
 
 }  // namespace
 
-struct kw_ctx {
-  int device;
-};
 struct kw_elastic {
   std::unique_ptr<ElBase> impl;
 };
-
-#define KW_GUARD_BEGIN try {
-#define KW_GUARD_END                \
-  }                                 \
-  catch (const KwError& e) {        \
-    g_err = e.msg;                  \
-    return e.code;                  \
-  }                                 \
-  catch (const std::exception& e) { \
-    g_err = e.what();               \
-    return KW_ERR_INVALID;          \
-  }                                 \
-  catch (...) {                     \
-    g_err = "unknown error";        \
-    return KW_ERR_INVALID;          \
-  }
 
 extern "C" {
 

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -252,3 +253,26 @@ inline void unshifted_derivative(long long n, double h, std::vector<double>& re,
 }
 
 }  // namespace kwh
+
+// Device context behind kw_ctx: owns nothing but the device selection.
+struct kw_ctx {
+  int device;
+};
+
+// Every C entry point runs its body inside these: KwError carries the status
+// code, anything else maps to KW_ERR_INVALID; the message goes to kw_last_error().
+#define KW_GUARD_BEGIN try {
+#define KW_GUARD_END                     \
+  }                                      \
+  catch (const kwh::KwError& e) {        \
+    kwh::g_err = e.msg;                  \
+    return e.code;                       \
+  }                                      \
+  catch (const std::exception& e) {      \
+    kwh::g_err = e.what();               \
+    return KW_ERR_INVALID;               \
+  }                                      \
+  catch (...) {                          \
+    kwh::g_err = "unknown error";        \
+    return KW_ERR_INVALID;               \
+  }

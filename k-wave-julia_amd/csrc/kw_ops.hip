@@ -13,10 +13,6 @@
 using namespace kw;
 using namespace kwh;
 
-struct kw_ctx {
-  int device;
-};
-
 namespace {
 
 template <typename T>
@@ -72,22 +68,6 @@ void spectral_gradient(int dev, int ndim, const int64_t* N, const double* d, int
 }
 
 }  // namespace
-
-#define KW_GUARD_BEGIN try {
-#define KW_GUARD_END                \
-  }                                 \
-  catch (const KwError& e) {        \
-    g_err = e.msg;                  \
-    return e.code;                  \
-  }                                 \
-  catch (const std::exception& e) { \
-    g_err = e.what();               \
-    return KW_ERR_INVALID;          \
-  }                                 \
-  catch (...) {                     \
-    g_err = "unknown error";        \
-    return KW_ERR_INVALID;          \
-  }
 
 extern "C" {
 

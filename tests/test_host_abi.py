@@ -133,3 +133,18 @@ def test_tuning_table_text_round_trip():
     finally:
         tuning.clear_table()
     assert tuning.table() == {}
+
+
+def test_errors_from_every_translation_unit_reach_kw_last_error():
+    """The operator and elastic entry points report through the same thread-local message."""
+    import kwave_amd as K
+    from kwave_amd import operators
+    with pytest.raises(K.KWaveHIPError, match="unknown axis table"):
+        lib = _lib.load()
+        out = (C.c_double * 8)()
+        _lib.check(lib.kw_test_axis_table(8, 1e-4, 0, 0.0, 1e-8, 99, out))
+    with pytest.raises(K.KWaveHIPError, match="pml_size out of range"):
+        operators.axis_table(8, 1e-4, "pml", pml_size=9, pml_alpha=2.0, dt=1e-8)
+    lib = _lib.load()
+    assert lib.kw_elastic_create(None, None, None) == _lib.KW_ERR_INVALID
+    assert b"null argument" in lib.kw_last_error()
