@@ -17,6 +17,7 @@ CASES = [
     ("stokes_p0_psrc", (64, 64)),
     ("hetc_nonlin_psrc_urec", (48, 96)),
     ("usrc_urec_irec", (64, 32)),
+    ("hetc_hetr_absorb_psrc", (45, 27)),     # odd radial axis: r = 0 sits on a grid point
 ]
 
 

@@ -105,3 +105,20 @@ def test_autotune_batch_then_parity():
         ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
         for k in outs[i].keys():
             assert rel_l2(outs[i][k], ref[k]) <= 1e-10, (i, k)
+
+
+def test_batch_odd_and_long_lines():
+    """Odd line lengths (unpaired strided loads) and 768/1024-point lines (XCD-ordered
+    strided and spectral passes) in one batch, FP32 and FP64."""
+    sims, kw = _sims([("hetc_absorb_psrc", (45, 27)), ("stokes_p0_psrc", (75, 25)),
+                      ("hetc_hetr_nonlin_psrc_urec", (45, 27)), ("psrc", (16, 768)),
+                      ("hetc_absorb_psrc", (24, 1024))], nt=30)
+    for dtype, tol in ((np.float64, 1e-10), (np.float32, 1e-4)):
+        print(f"\n[case] batch odd + long lines {dtype.__name__}", flush=True)
+        outs = K.kspace_first_order_batch([tuple(copy.deepcopy(s)) for s in sims], data_cast=dtype,
+                                          use_graph=True, **kw)
+        for i, (kg, med, src, sen) in enumerate(sims):
+            ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
+            for k in outs[i].keys():
+                e = rel_l2(outs[i][k], ref[k])
+                assert e <= tol, (i, k, e)
