@@ -92,7 +92,10 @@ __device__ __forceinline__ Cx<T> el_med(const LineCtx<T, N, V>& L, const T* m, T
   return m ? el_ld(L, m, q) : Cx<T>{s, s};
 }
 
-template <typename T, int N, bool CONTIG>
+// MODE is a template parameter so that each epilogue gets its own register
+// allocation (the whole-line loads of the velocity and shear epilogues would
+// otherwise cost the derivative and normal-stress passes occupancy).
+template <typename T, int N, bool CONTIG, int MODE>
 __global__ void __launch_bounds__(512) k_el_pass(ElArgs<T> a) {
   constexpr int V = CONTIG ? 1 : 0;
   constexpr int E = Plan<N, V>::E;
@@ -118,7 +121,7 @@ __global__ void __launch_bounds__(512) k_el_pass(ElArgs<T> a) {
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.D[L.n(q)]);
   line_fft<T, N, true, V>(v, lds, tw, L.t);
 
-  if (a.mode == EL_DERIV) {
+  if constexpr (MODE == EL_DERIV) {
     if (a.accumulate) {
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = cadd(v[q], el_ld(L, a.out, q));
@@ -126,25 +129,36 @@ __global__ void __launch_bounds__(512) k_el_pass(ElArgs<T> a) {
     L.store(a.out, v);
     return;
   }
-  if (a.mode == EL_VEL) {
+  if constexpr (MODE == EL_VEL) {
+    // whole-line loads first (more loads in flight), then the update
+    Cx<T> u[E];
+    L.load(a.v, u);
+    if (a.acc) {
+      Cx<T> g[E];
+      L.load(a.acc, g);
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = cadd(v[q], g[q]);
+    }
+    if (a.rho) {
+      Cx<T> r0[E], r1[E];
+      L.load_map(a.rho, r0, false, (T)0, (T)1);
+      L.load_map_next(a.rho, r1, false, (T)0, (T)1);
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        v[q].x *= fdiv((T)2 * a.dt, r0[q].x + r1[q].x);
+        v[q].y *= fdiv((T)2 * a.dt, r0[q].y + r1[q].y);
+      }
+    } else {
+      const T f = a.dt / a.rho_s;
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = cscale(v[q], f);
+    }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      Cx<T> g = v[q];
-      if (a.acc) g = cadd(g, el_ld(L, a.acc, q));
-      Cx<T> f;
-      if (a.rho) {
-        const Cx<T> r0 = el_ld(L, a.rho, q);
-        const unsigned nx = L.nx(q);
-        const T* __restrict__ rb = a.rho + L.wgb;
-        const T r1a = L.va ? at32(rb, L.oa + nx) : (T)1, r1b = L.vb ? at32(rb, L.ob + nx) : (T)1;
-        f = Cx<T>{fdiv((T)2 * a.dt, (L.va ? r0.x : (T)1) + r1a), fdiv((T)2 * a.dt, (L.vb ? r0.y : (T)1) + r1b)};
-      } else {
-        f = Cx<T>{a.dt / a.rho_s, a.dt / a.rho_s};
-      }
       const T sg = a.pml_sg[L.n(q)];
-      const Cx<T> u = el_ld(L, a.v, q);
-      el_st(L, a.v, q, Cx<T>{sg * (sg * u.x + f.x * g.x), sg * (sg * u.y + f.y * g.y)});
+      u[q] = Cx<T>{sg * (sg * u[q].x + v[q].x), sg * (sg * u[q].y + v[q].y)};
     }
+    L.store(a.v, u);
     return;
   }
 
@@ -162,7 +176,7 @@ __global__ void __launch_bounds__(512) k_el_pass(ElArgs<T> a) {
       if (L.vb) pb[d] = a.pml[d][cb[d]];
     }
   }
-  if (a.mode == EL_NORMAL) {
+  if constexpr (MODE == EL_NORMAL) {
     const bool three = a.ndim == 3;
 #pragma unroll
     for (int q = 0; q < E; ++q) {
@@ -188,17 +202,30 @@ __global__ void __launch_bounds__(512) k_el_pass(ElArgs<T> a) {
   // EL_SHEAR
   const T qa = a.axis2 == 0 ? pa[0] : (a.axis2 == 1 ? pa[1] : pa[2]);
   const T qb = a.axis2 == 0 ? pb[0] : (a.axis2 == 1 ? pb[1] : pb[2]);
+  Cx<T> s[E];
+  L.load(a.s[0], s);
+  if (a.acc) {
+    Cx<T> g[E];
+    L.load(a.acc, g);
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cadd(v[q], g[q]);
+  }
+  if (a.mu) {
+    Cx<T> m[E];
+    L.load(a.mu, m);
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = Cx<T>{m[q].x * v[q].x, m[q].y * v[q].y};
+  } else {
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cscale(v[q], a.mu_s);
+  }
 #pragma unroll
   for (int q = 0; q < E; ++q) {
-    Cx<T> g = v[q];
-    if (a.acc) g = cadd(g, el_ld(L, a.acc, q));
-    const Cx<T> mu = el_med(L, a.mu, a.mu_s, q);
     const T pn = pml_line[L.n(q)];
     const T fa = pn * qa, fb = pn * qb;
-    const Cx<T> s = el_ld(L, a.s[0], q);
-    el_st(L, a.s[0], q,
-          Cx<T>{fa * (s.x + a.dt * a.ks * mu.x * g.x), fb * (s.y + a.dt * a.ks * mu.y * g.y)});
+    s[q] = Cx<T>{fa * (s[q].x + a.dt * a.ks * v[q].x), fb * (s[q].y + a.dt * a.ks * v[q].y)};
   }
+  L.store(a.s[0], s);
 }
 
 // p = -(sum of normal stresses) / ndim at the sensor points only

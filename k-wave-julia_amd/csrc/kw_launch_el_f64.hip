@@ -6,15 +6,24 @@ namespace kw {
 template <>
 hipError_t launch_el<double>(int n, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const ElArgs<double>& a) {
   switch (n) {
-#define X(n_)                                                              \
-  case n_:                                                                 \
-    if (contig)                                                            \
-      hipLaunchKernelGGL((k_el_pass<double, n_, true>), g, b, sh, s, a);      \
-    else                                                                   \
-      hipLaunchKernelGGL((k_el_pass<double, n_, false>), g, b, sh, s, a);     \
+#define M(n_, c_)                                                                            \
+  switch (a.mode) {                                                                          \
+    case EL_DERIV: hipLaunchKernelGGL((k_el_pass<double, n_, c_, EL_DERIV>), g, b, sh, s, a); break;   \
+    case EL_VEL: hipLaunchKernelGGL((k_el_pass<double, n_, c_, EL_VEL>), g, b, sh, s, a); break;       \
+    case EL_NORMAL: hipLaunchKernelGGL((k_el_pass<double, n_, c_, EL_NORMAL>), g, b, sh, s, a); break; \
+    case EL_SHEAR: hipLaunchKernelGGL((k_el_pass<double, n_, c_, EL_SHEAR>), g, b, sh, s, a); break;   \
+    default: return hipErrorInvalidValue;                                                     \
+  }
+#define X(n_)      \
+  case n_:         \
+    if (contig)    \
+      M(n_, true)  \
+    else           \
+      M(n_, false) \
     return hipGetLastError();
     KW_SIZES(X)
 #undef X
+#undef M
   }
   return hipErrorInvalidValue;
 }
