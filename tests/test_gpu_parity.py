@@ -142,11 +142,13 @@ def test_sensor_post_filters(angles):
         _compare(out, ref, tol, f"filters-{angles}-{dt.__name__}")
 
 
-def test_fp16_maps():
+@pytest.mark.parametrize("shape", [(32, 24, 16), (27, 25, 15)], ids=["even", "odd"])
+def test_fp16_maps(shape):
     """FP16 map storage: arithmetic error vs the oracle fed the dequantized maps
-    (<= 1e-4) and the total error vs the unquantized maps (reported)."""
-    print("\n[case] fp16 maps", flush=True)
-    kg, med, src, sen, kw = scenario("hetc_hetr_absorb_nonlin_psrc", (32, 24, 16), nt=40)
+    (<= 1e-4) and the total error vs the unquantized maps (reported).  The odd
+    shape takes the per-line (unpaired) FP16 loads in every pass."""
+    print(f"\n[case] fp16 maps {shape}", flush=True)
+    kg, med, src, sen, kw = scenario("hetc_hetr_absorb_nonlin_psrc", shape, nt=40)
     cq, _ = K.quantize_map(med.sound_speed)
     rq, _ = K.quantize_map(med.density)
     medq = K.KWaveMedium(sound_speed=cq, density=rq, alpha_coeff=med.alpha_coeff, alpha_power=med.alpha_power,
