@@ -66,12 +66,12 @@ def cpu_baseline(n_steps_max: int = 20, budget_s: float = 20.0):
     """Oracle (FP32 restatement, scipy.fft threads) on the same workload, truncated."""
     import numpy as np
     from oracle import kwave_oracle as O
-    from kwave_amd.configs import config2
+    from kwave_amd.configs import ellipsoids_3d
 
     workers = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     workers = min(workers, 16)
     O.set_fft_workers(workers)
-    kg, med, src, sen, kw = config2(Nt=1200)
+    kg, med, src, sen, kw = ellipsoids_3d(Nt=1200)
     stamps = []
     t_start = time.perf_counter()
 
@@ -99,13 +99,13 @@ def cpu_baseline(n_steps_max: int = 20, budget_s: float = 20.0):
 
 
 def bench_batch(a):
-    """Configuration 3: 64 uneven 2-D firings; batched (one launch per pass per line
+    """64 uneven 2-D firings on 256^2..1024^2 grids; batched (one launch per pass per line
     length, hipGraph step) against the same simulations run one after another."""
     import numpy as np
     import kwave_amd as K
-    from kwave_amd.configs import config3
+    from kwave_amd.configs import uneven_2d_batch
 
-    sims, kw = config3(Nt=a.warmup + a.steps)
+    sims, kw = uneven_2d_batch(Nt=a.warmup + a.steps)
     descs, keep = [], []
     for s in sims:
         d, k, _ = K.build_desc(*s, data_cast=np.float32, **kw)
@@ -174,9 +174,9 @@ def bench_adjoint(a):
     """
     import numpy as np
     import kwave_amd as K
-    from kwave_amd.configs import config5
+    from kwave_amd.configs import ring_2d_adjoint
 
-    kg, c_true, c_start, firings, kw = config5()
+    kg, c_true, c_start, firings, kw = ring_2d_adjoint()
     firings = firings[: a.firings]
     med_true = K.KWaveMedium(sound_speed=c_true, density=1000.0)
     med0 = K.KWaveMedium(sound_speed=c_start, density=1000.0)
@@ -233,9 +233,9 @@ def bench_c4(a):
     """Precision form: 512^3 layered medium, FP16 map storage, FP32 arithmetic, two sensor sets."""
     import numpy as np
     import kwave_amd as K
-    from kwave_amd.configs import config4
+    from kwave_amd.configs import layered_3d_fp16
 
-    kg, med, src, sensors, kw = config4(Nt=a.warmup + a.steps + a.profile_steps)
+    kg, med, src, sensors, kw = layered_3d_fp16(Nt=a.warmup + a.steps + a.profile_steps)
     if a.c4_maps == "fp32":
         kw["medium_storage"] = None   # comparison run: maps in the kernel precision
     res = {}
@@ -273,12 +273,12 @@ def cpu_baseline_elastic(n: int = 128, n_steps_max: int = 12, budget_s: float = 
     """Oracle elastic restatement (FP32, scipy.fft threads) on an n^3 cut of the workload."""
     import numpy as np
     from oracle import kwave_oracle as O
-    from kwave_amd.configs import config_elastic
+    from kwave_amd.configs import two_layer_elastic_3d
 
     workers = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     workers = min(workers, 16)
     O.set_fft_workers(workers)
-    kg, med, src, sen, kw = config_elastic(N=n, Nt=200)
+    kg, med, src, sen, kw = two_layer_elastic_3d(N=n, Nt=200)
     stamps = []
 
     class Stop(Exception):
@@ -304,11 +304,11 @@ def bench_elastic(a):
     """Elastic stress-velocity PSTD (pstd_elastic_3d): 3-D two-layer 256^3 FP32 workload."""
     import numpy as np
     import kwave_amd as K
-    from kwave_amd.configs import config_elastic, elastic_pass_bytes
+    from kwave_amd.configs import two_layer_elastic_3d, elastic_pass_bytes
     from kwave_amd.elastic import ElasticSimulation, build_elastic_desc
 
     N = a.N
-    kg, med, src, sen, kw = config_elastic(N=N, Nt=a.warmup + a.steps + a.profile_steps)
+    kg, med, src, sen, kw = two_layer_elastic_3d(N=N, Nt=a.warmup + a.steps + a.profile_steps)
     desc, keep, _ = build_elastic_desc(kg, med, src, sen, data_cast=np.float32, **kw)
     sim = ElasticSimulation(desc, keep)
     del keep
@@ -402,11 +402,11 @@ def main():
 
     import numpy as np
     import kwave_amd as K
-    from kwave_amd.configs import config2, config2_bytes_per_voxel_update
+    from kwave_amd.configs import ellipsoids_3d, ellipsoids_3d_bytes_per_voxel_update
 
     t0 = time.perf_counter()
     nt = a.warmup + a.steps + a.profile_steps
-    kg, med, src, sen, kw = config2(N=a.N, Nt=nt)
+    kg, med, src, sen, kw = ellipsoids_3d(N=a.N, Nt=nt)
     desc, keep, info = K.build_desc(kg, med, src, sen, pml_size=kw["pml_size"], pml_alpha=kw["pml_alpha"],
                                     smooth_p0=False, data_cast=np.float32)
     sim = K.Simulation(desc, keep)
@@ -456,7 +456,7 @@ def main():
         roof = {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dname, N),
                 "bytes_per_launch": bpv * vox, "avg_launch_ms": dstat["ms"]}
-    step_b = config2_bytes_per_voxel_update(4)
+    step_b = ellipsoids_3d_bytes_per_voxel_update(4)
     out = {
         "metric": METRIC,
         "value": value,

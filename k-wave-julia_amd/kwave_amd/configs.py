@@ -35,7 +35,7 @@ class SplitMix64:
         return lo + int(self.uniform() * (hi - lo + 1))
 
 
-def config1(data_cast=np.float64):
+def homogeneous_2d(data_cast=np.float64):
     """2-D 128x128, dx = 0.1 mm, homogeneous lossless; disc p source, circle sensor."""
     kg = KWaveGrid(128, 1e-4, 128, 1e-4)
     make_time(kg, 1500.0)
@@ -49,7 +49,7 @@ def config1(data_cast=np.float64):
     return kg, med, src, sen, dict(pml_size=20, pml_alpha=2.0, data_cast=data_cast)
 
 
-def config2(N: int = 256, Nt: int = 1200, pml: int = 20, amplitude: float = 1e6, seed: int = 42):
+def ellipsoids_3d(N: int = 256, Nt: int = 1200, pml: int = 20, amplitude: float = 1e6, seed: int = 42):
     """3-D heterogeneous absorbing nonlinear FP32 workload (N^3, dx = 0.1 mm).
 
     c0 = 1500 m/s plus 8 axis-aligned ellipsoids (centres in [pml+1, N-pml]^3,
@@ -98,12 +98,12 @@ def config2(N: int = 256, Nt: int = 1200, pml: int = 20, amplitude: float = 1e6,
     return kg, med, src, sen, dict(pml_size=pml, pml_alpha=2.0, data_cast=np.float32, smooth_p0=False)
 
 
-def config2_bytes_per_voxel_update(precision: int = 4) -> int:
+def ellipsoids_3d_bytes_per_voxel_update(precision: int = 4) -> int:
     """Compulsory bytes per voxel-update: 7 evolving fields read+written, 2 maps read."""
     return 2 * 7 * precision + 2 * precision
 
 
-def config3(n_sims: int = 64, Nt: int = 1500, seed: int = 42, sizes=(256, 384, 512, 768, 1024)):
+def uneven_2d_batch(n_sims: int = 64, Nt: int = 1500, seed: int = 42, sizes=(256, 384, 512, 768, 1024)):
     """Batch of 2-D firings of a 128-element linear array (pitch 3, width 2 cells).
 
     Per simulation: an N x N grid with N drawn (seeded) from `sizes`, dx = 0.1 mm;
@@ -152,7 +152,7 @@ def config3(n_sims: int = 64, Nt: int = 1500, seed: int = 42, sizes=(256, 384, 5
     return sims, dict(pml_size=20, pml_alpha=2.0, smooth_p0=False)
 
 
-def config4(N: int = 512, Nt: int = 2000, pml: int = 24, amplitude: float = 1e6, seed: int = 42,
+def layered_3d_fp16(N: int = 512, Nt: int = 2000, pml: int = 24, amplitude: float = 1e6, seed: int = 42,
             n_points: int = 4096):
     """3-D layered FP16-map workload (N^3, dx = 0.2 mm, PML 24).
 
@@ -198,7 +198,7 @@ def config4(N: int = 512, Nt: int = 2000, pml: int = 24, amplitude: float = 1e6,
                                               medium_storage=np.float16)
 
 
-def config5(N: int = 512, Nt: int = 1600, n_elements: int = 128, radius: float = 220.0, seed: int = 42):
+def ring_2d_adjoint(N: int = 512, Nt: int = 1600, n_elements: int = 128, radius: float = 220.0, seed: int = 42):
     """Adjoint workload: 2-D N x N, dx = 0.1 mm, rho0 = 1000, lossless linear.
 
     True model: c0 = 1500 plus 6 seeded discs (radius integer in [10,40],
@@ -240,7 +240,7 @@ def config5(N: int = 512, Nt: int = 1600, n_elements: int = 128, radius: float =
     return kg, c_true, c_start, firings, dict(pml_size=20, pml_alpha=2.0, smooth_p0=False)
 
 
-def config_elastic(N: int = 256, Nt: int = 600, pml: int = 10, seed: int = 42):
+def two_layer_elastic_3d(N: int = 256, Nt: int = 600, pml: int = 10, seed: int = 42):
     """3-D two-layer elastic workload (N^3, dx = 0.1 mm, PML 10) for pstd_elastic_3d.
 
     z < N/2: fluid-like layer (cp 1540 m/s, cs 0, rho 1000); z >= N/2: solid

@@ -3,7 +3,7 @@
 The fixtures come from the exact per-mode recurrence of the reference scheme
 (oracle/closed_form.py: homogeneous medium, pml_size = 0, linear EOS), which
 uses no part of the oracle's or the product's FFT plumbing, plus the
-configuration-1 inputs (tone burst, disc and circle masks) that the
+128^2 example inputs (tone burst, disc and circle masks) that the
 reference's own tests constrain (test_signal.jl, test_geometry.jl).
 
     python tests/golden/make_golden.py
@@ -65,7 +65,7 @@ def recurrence_case(name, shape, nt, mode=None, alpha=0.0, y=1.5, p0=False, seed
 
 
 def c1_inputs():
-    """Configuration-1 inputs: KWaveGrid(128,1e-4,128,1e-4), make_time!(kgrid,1500)."""
+    """Inputs of the 128^2 example: KWaveGrid(128,1e-4,128,1e-4), make_time!(kgrid,1500)."""
     dt = 0.3 * 1e-4 / 1500.0
     fs = 1 / dt
     n = int(round((5 / 1e6) / dt))

@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 import kwave_amd as K
-from kwave_amd.configs import config2, config4
+from kwave_amd.configs import ellipsoids_3d, layered_3d_fp16
 
 from cases import rel_l2
 
@@ -34,7 +34,7 @@ def _near_plane_sensor(N, z, record):
 
 def test_c2_fp32_vs_fp64_256cube():
     print("\n[case] 256^3 heterogeneous absorbing nonlinear, FP32 vs FP64, 200 steps", flush=True)
-    kg, med, src, _, kw = config2(N=256, Nt=200)
+    kg, med, src, _, kw = ellipsoids_3d(N=256, Nt=200)
     kw.pop("data_cast")
     sen = _near_plane_sensor(256, 60, ["p", "p_max", "p_min", "p_rms", "p_final"])
     o64 = K.kspace_first_order(copy.deepcopy(kg), med, src, sen, data_cast=np.float64, **kw)
@@ -48,7 +48,7 @@ def test_c2_fp32_vs_fp64_256cube():
 
 def test_superposition_256cube():
     print("\n[case] 256^3 linear lossless superposition, FP64, 150 steps", flush=True)
-    kg, med, src, _, kw = config2(N=256, Nt=150)
+    kg, med, src, _, kw = ellipsoids_3d(N=256, Nt=150)
     kw.pop("data_cast")
     lin = K.KWaveMedium(sound_speed=med.sound_speed, density=med.density)
     pm = src.p_mask
@@ -73,7 +73,7 @@ def test_superposition_256cube():
 
 def test_c4_fp16_maps_512cube():
     print("\n[case] 512^3 FP16 maps: FP32 arithmetic vs FP64 on the dequantised maps, 30 steps", flush=True)
-    kg, med, src, sensors, kw = config4(Nt=30)
+    kg, med, src, sensors, kw = layered_3d_fp16(Nt=30)
     kw = dict(kw)
     kw.pop("medium_storage")
     sen = K.KWaveSensor(mask=src.p_mask, record=["p", "p_final"])

@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the FP64 oracle.
 
 FP64 kernels must match the oracle to <= 1e-10 relative L2 and FP32 kernels
-to <= 1e-4, on every recorded array.
+to <= 1e-4 (the acceptance tolerances), on every recorded array.
 """
 import copy
 import sys

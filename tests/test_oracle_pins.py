@@ -43,7 +43,7 @@ def test_grid_vectors_and_time():
     assert math.isclose(dt, 0.3 * 1e-4 / 1500.0)
     dt2, nt2 = O.make_time((128, 128), (1e-4, 1e-4), 1500.0, t_end=1e-5)
     assert nt2 == math.ceil(1e-5 / dt2) + 1
-    # configuration 1: default t_end = 2*diag/c -> Nt = 1208
+    # 128^2 grid, dx 0.1 mm, c 1500: default t_end = 2*diag/c -> Nt = 1208 (grid.jl:320-352)
     assert nt == 1208
 
 
@@ -80,7 +80,7 @@ def test_pml_reference_assertions_and_closed_form():
 # ---------------------------------------------------------------- absorption
 def test_db2neper_and_absorption_prefactors():
     a = O.db2neper(0.75, 1.5)
-    # conversion.jl:34 with the reference's 1e-2 factor (5.48e-14)
+    # conversion.jl:34 with the reference's 1e-2 factor: alpha 0.75, y 1.5 -> 5.48e-14
     assert a == pytest.approx(0.75 * 1e-2 / 8.685889638065035 * (2 * math.pi * 1e6) ** -1.5, rel=1e-14)
     assert a == pytest.approx(5.48e-14, rel=2e-3)
     assert O.db2neper(1.0, 1.0) > O.db2neper(1.0, 2.0)
@@ -121,7 +121,7 @@ def test_masks_reference_assertions_and_brute_force():
     g = np.load(os.path.join(GOLD, "c1_inputs.npz"))
     d1 = O.make_disc(128, 128, 64, 64, 5)
     assert np.array_equal(O._findall(d1), g["disc_idx"]) and d1.sum() == 81
-    assert O.make_circle(128, 128, 64, 64, 40).sum() == 228           # configuration 1 sensor
+    assert O.make_circle(128, 128, 64, 64, 40).sum() == 228           # 228-cell circle sensor of the 128^2 example
     for f, args in ((O.make_disc, (40, 30, 17, 11, 6)), (O.make_circle, (40, 30, 17, 11, 9)),
                     (O.make_ball, (20, 18, 16, 9, 8, 7, 5))):
         assert np.array_equal(f(*args), getattr(K, f.__name__)(*args))
