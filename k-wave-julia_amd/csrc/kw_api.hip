@@ -809,6 +809,9 @@ class Sim final : public SimBase {
     for (long long v : idx)
       if (v < 0 || v >= nvox_) raise(KW_ERR_INVALID, "sensor index out of range");
     upload(sidx_, idx);
+    // a mask covering the whole grid records voxel j in row j: the recorder skips the index stream
+    dense_sensor_ = nsens_ == nvox_;
+    for (long long j = 0; j < nsens_ && dense_sensor_; ++j) dense_sensor_ = idx[j] == j;
     auto fill_t = [&](DevBuf& b, long long n, double v) {
       upload(b, std::vector<T>(n, (T)v));
     };
@@ -963,7 +966,7 @@ class Sim final : public SimBase {
     ra_.p = p_.as<T>();
     for (int a = 0; a < 3; ++a) ra_.u[a] = a < nd_ ? u_[a].as<T>() : nullptr;
     ra_.ndim = nd_;
-    ra_.idx = sidx_.as<long long>();
+    ra_.idx = dense_sensor_ ? nullptr : sidx_.as<long long>();
     ra_.n = (int)nsens_;
     ra_.Nt = (int)Nt_;
     ra_.rec = rec_;
@@ -1206,6 +1209,7 @@ class Sim final : public SimBase {
   long long nsens_ = 0;
   uint32_t rec_ = 0;
   DevBuf sidx_, p_ts_, u_ts_[3], p_max_, p_min_, p_rms_, p_final_, u_max_, u_rms_, u_final_, I_avg_, I_max_;
+  bool dense_sensor_ = false;
   DevBuf tr_vals_;
   // adjoint
   bool het_c_ = false, nonlinear_ = false, has_p0_ = false, absorbing_ = false, no_record_ = false;

@@ -882,7 +882,7 @@ struct RecArgs {
   const T* p;
   const T* u[3];
   int ndim;
-  const long long* idx;
+  const long long* idx;  // sensor voxel of each row, or null when the mask is the whole grid
   int n;
   int t;       // 1-based
   int Nt;
@@ -902,7 +902,7 @@ struct RecArgs {
 
 template <typename T>
 __device__ __forceinline__ void record_body(const RecArgs<T>& a, int j, int t_index) {
-  const long long i = a.idx[j];
+  const long long i = a.idx ? a.idx[j] : (long long)j;   // null: every voxel, in order
   const T pv = a.p[i];
   const long long o = (long long)(t_index - 1) * a.n + j;
   if (a.rec & REC_P) a.p_ts[o] = pv;
@@ -911,7 +911,9 @@ __device__ __forceinline__ void record_body(const RecArgs<T>& a, int j, int t_in
   if (a.rec & REC_P_RMS) a.p_rms[j] += (double)(pv * pv);
   if ((a.rec & REC_P_FINAL) && t_index == a.Nt) a.p_final[j] = pv;
   T uv[3] = {0, 0, 0};
-  for (int d = 0; d < a.ndim; ++d) uv[d] = a.u[d][i];
+  // velocities only when a velocity or intensity field is recorded
+  if (a.rec & (REC_UX | REC_UY | REC_UZ | REC_U_MAX | REC_U_RMS | REC_U_FINAL | REC_I_AVG | REC_I_MAX))
+    for (int d = 0; d < a.ndim; ++d) uv[d] = a.u[d][i];
   if (a.rec & REC_UX) a.u_ts[0][o] = uv[0];
   if ((a.rec & REC_UY) && a.ndim > 1) a.u_ts[1][o] = uv[1];
   if ((a.rec & REC_UZ) && a.ndim > 2) a.u_ts[2][o] = uv[2];

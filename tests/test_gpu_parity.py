@@ -159,3 +159,18 @@ def test_fp16_maps(shape):
     _compare(out, ref_q, 1e-4, "fp16-arith")
     for k in out.keys():
         print(f"    fp16-total {k:8s} rel-L2 {rel_l2(out[k], ref[k]):.3e}", flush=True)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+def test_whole_grid_sensor(dtype):
+    """A mask covering every voxel (recorded without an index stream) vs the oracle."""
+    print(f"\n[case] whole-grid sensor mask {dtype.__name__}", flush=True)
+    kg, med, src, sen, kw = scenario("hetc_hetr_nonlin_psrc", (24, 16, 20), nt=30)
+    full = K.KWaveSensor(mask=np.ones((24, 16, 20), dtype=bool),
+                         record=["p", "p_max", "p_min", "p_rms", "p_final", "ux", "uz", "u_max"])
+    ref = O.kspace_first_order(copy.deepcopy(kg), med, src, full, **kw)
+    out = K.kspace_first_order(copy.deepcopy(kg), med, src, full, data_cast=dtype, **kw)
+    _compare(out, ref, TOL[dtype], f"whole-grid/{dtype.__name__}")
+    outs = K.kspace_first_order_batch([(copy.deepcopy(kg), med, src, full)] * 2, data_cast=dtype, **kw)
+    for o in outs:
+        _compare(o, ref, TOL[dtype], f"whole-grid-batch/{dtype.__name__}")
