@@ -612,7 +612,7 @@ class Sim final : public SimBase {
   PassCfg pass_cfg(long long n, bool contig, long long lines, long long outer, int cls) {
     PassCfg c;
     c.n = (int)n;
-    const int tpl = plan_tpl((int)n, contig ? 1 : 0);
+    const int tpl = plan_tpl((int)n, cls == TC_C2R ? 2 : (contig ? 1 : 0));
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
     // strided lines: at least 16 line pairs per workgroup (128-byte rows in FP32)
     // when that fits 512 threads; contiguous lines: 256 threads

@@ -97,7 +97,7 @@ __device__ __forceinline__ Cx<T> el_med(const LineCtx<T, N, V>& L, const T* m, T
 // otherwise cost the derivative and normal-stress passes occupancy).
 template <typename T, int N, bool CONTIG, int MODE>
 __global__ void __launch_bounds__(512) k_el_pass(ElArgs<T> a) {
-  constexpr int V = CONTIG ? 1 : 0;
+  constexpr int V = CONTIG ? 2 : 0;  // contiguous lines: plan family 2 (fewer, longer rows per thread)
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(512) k_el_pass(ElArgs<T> a) {
   geo.x_blocks = a.x_blocks;
   int lpw;
   const LineCtx<T, N, V> L =
-      make_ctx<T, N, CONTIG>(geo, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), lpw);
+      make_ctx<T, N, CONTIG, V>(geo, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), lpw);
   Cx<T>* lds = lds_all + L.c * P;
   const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(a.tw, L.t);
 

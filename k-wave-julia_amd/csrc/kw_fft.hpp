@@ -232,7 +232,8 @@ struct Radices {};
 // V = 0: plans of the strided and spectral passes (lines spread across
 // workgroup lanes, so more lines per workgroup = wider coalesced rows);
 // V = 1: plans of the contiguous x-line passes, which may trade a larger
-// TPL (fewer registers, more waves) for an extra LDS exchange.
+// TPL (fewer registers, more waves) for extra LDS exchanges; V = 2 (below)
+// for the complex-to-real x pass.
 template <int N, int V = 0>
 struct Plan;
 #define KW_PLAN(N_, TPL_, ...)                     \
@@ -297,6 +298,18 @@ struct Plan<512, 1> {
 };
 template <>
 struct Plan<256, 1> {
+  static constexpr int N = 256;
+  static constexpr int TPL = 64;
+  static constexpr int E = 4;
+  using R = Radices<4, 4, 4, 4>;
+};
+// V = 2: the complex-to-real pass along x (and the adjoint's r2c stage),
+// which carry fewer live arrays than the directional x pass and run faster
+// with more elements per thread.
+template <int N>
+struct Plan<N, 2> : Plan<N, 1> {};
+template <>
+struct Plan<256, 2> {
   static constexpr int N = 256;
   static constexpr int TPL = 32;
   static constexpr int E = 8;

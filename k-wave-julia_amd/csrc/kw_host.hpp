@@ -174,12 +174,13 @@ struct PassCfg {
 
 // Lines per workgroup of a line pass: 256 threads on contiguous lines, >= 16
 // line pairs up to 512 threads on strided lines, halved until the lines fit
-// the workgroup's LDS and to the lines that exist.
+// the workgroup's LDS and to the lines that exist.  These passes (elastic,
+// operator tests) use plan family 2 on contiguous lines.
 inline PassCfg line_pass_cfg(long long n, bool contig, long long lines, long long outer, size_t cx_bytes,
                              size_t lds_max) {
   PassCfg c;
   c.n = (int)n;
-  const int tpl = kw::plan_tpl((int)n, contig ? 1 : 0);
+  const int tpl = kw::plan_tpl((int)n, contig ? 2 : 0);
   const size_t line_bytes = (size_t)kw::plan_pitch((int)n) * cx_bytes;
   int lpw = (contig ? 256 : std::max(256, std::min(512, 16 * tpl))) / tpl;
   while (lpw > 1 && (size_t)lpw * line_bytes > lds_max) lpw /= 2;

@@ -288,9 +288,10 @@ struct LineCtx {
   }
 };
 
-template <typename T, int N, bool CONTIG>
-__device__ __forceinline__ LineCtx<T, N, CONTIG ? 1 : 0> make_ctx(const PassArgs<T>& a, int wg, int& lpw_out) {
-  constexpr int V = CONTIG ? 1 : 0;
+// VV selects the plan family (default: 1 for contiguous x lines, 0 for strided).
+template <typename T, int N, bool CONTIG, int VV = (CONTIG ? 1 : 0)>
+__device__ __forceinline__ LineCtx<T, N, VV> make_ctx(const PassArgs<T>& a, int wg, int& lpw_out) {
+  constexpr int V = VV;
   constexpr int TPL = Plan<N, V>::TPL;
   LineCtx<T, N, V> L;
   const int tid = threadIdx.x;
@@ -791,7 +792,7 @@ struct C2RArgs {
 
 template <typename T, int N>
 __device__ __forceinline__ void c2r_body(const C2RArgs<T>& a, int wg) {
-  constexpr int V = 1;  // contiguous x lines
+  constexpr int V = 2;  // contiguous x lines, complex-to-real plan family
   constexpr int TPL = Plan<N, V>::TPL;
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
@@ -1232,7 +1233,7 @@ __global__ void __launch_bounds__(512) k_adj_pass(AdjArgs<T> a) {
 // Real lines (pairs) of f -> half spectrum along x (the r2c stage alone).
 template <typename T, int N>
 __global__ void __launch_bounds__(256) k_r2c_x(const T* f, Cx<T>* S, int n_lines, int nhp, const Cx<T>* twt) {
-  constexpr int V = 1;  // contiguous x lines
+  constexpr int V = 2;  // contiguous x lines, complex-to-real plan family
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   constexpr int NH = N / 2 + 1;
@@ -1241,7 +1242,7 @@ __global__ void __launch_bounds__(256) k_r2c_x(const T* f, Cx<T>* S, int n_lines
   PassArgs<T> geo;
   geo.n_lines = n_lines;
   int lpw;
-  const LineCtx<T, N, V> L = make_ctx<T, N, true>(geo, blockIdx.x, lpw);
+  const LineCtx<T, N, V> L = make_ctx<T, N, true, V>(geo, blockIdx.x, lpw);
   Cx<T>* lds = lds_all + L.c * P;
   const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(twt, L.t);
   Cx<T> z[E];

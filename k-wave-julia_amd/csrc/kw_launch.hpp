@@ -20,7 +20,7 @@ inline unsigned adj_reduce_blocks(long long n) {
   const long long nb = (n + 255) / 256;
   return nb < 1 ? 1u : (nb < 1024 ? (unsigned)nb : 1024u);
 }
-int plan_tpl(int n, int variant = 0);  // variant 1: contiguous x-line passes
+int plan_tpl(int n, int variant = 0);  // variant 1: contiguous x-line passes, 2: complex-to-real x pass
 int plan_pitch(int n);
 
 // Return hipSuccess or the launch error; unsupported lengths give hipErrorInvalidValue.
