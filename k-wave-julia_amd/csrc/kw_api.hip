@@ -530,6 +530,9 @@ class Sim final : public SimBase {
       int v = 0;
       HIPC(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_));
       lds_max_ = v > 0 ? (size_t)v : 65536;
+      int cus = 0;
+      HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_));
+      fill_wgs_ = 2LL * std::max(cus, 1);
     }
     nvox_ = N_[0] * N_[1] * N_[2];
     NH_ = N_[0] / 2 + 1;
@@ -621,8 +624,15 @@ class Sim final : public SimBase {
     lpw = tuned_lpw((int)sizeof(T), cls, (int)n, lpw);
     c.req = lpw;
     // pairs that exist
-    const long long pairs = contig ? (lines + 1) / 2 : (lines + 1) / 2;
+    const long long pairs = (lines + 1) / 2;
     while (!batch_mode_ && lpw > 1 && lpw / 2 >= pairs) lpw /= 2;
+    // small grids (2-D, few lines): fewer lines per workgroup until the launch
+    // has two workgroups per CU, keeping at least one full wave per workgroup
+    if (!batch_mode_) {
+      const int min_lpw = std::max(1, 64 / tpl);
+      const long long o = contig ? 1 : std::max<long long>(outer, 1);
+      while (lpw / 2 >= min_lpw && ((pairs + lpw - 1) / lpw) * o < fill_wgs_) lpw /= 2;
+    }
     c.lpw = lpw;
     c.block = dim3(lpw * tpl);
     c.shmem = (size_t)lpw * line_bytes;
@@ -646,6 +656,10 @@ class Sim final : public SimBase {
     lpw = tuned_lpw((int)sizeof(T), cls, (int)n, lpw);
     c.req = lpw;
     while (!batch_mode_ && lpw > 1 && lpw / 2 >= NH_) lpw /= 2;
+    if (!batch_mode_) {   // as in pass_cfg: fill the device on small grids
+      const int min_lpw = std::max(1, 64 / tpl);
+      while (lpw / 2 >= min_lpw && ((NH_ + lpw - 1) / lpw) * std::max<long long>(outer, 1) < fill_wgs_) lpw /= 2;
+    }
     c.lpw = lpw;
     c.block = dim3(lpw * tpl);
     c.shmem = (size_t)lpw * line_bytes;
@@ -1216,6 +1230,7 @@ class Sim final : public SimBase {
   bool axisym_ = false;
   DevBuf inv_r_;
   size_t lds_max_ = 65536;   // LDS a workgroup may allocate on this device
+  long long fill_wgs_ = 512;  // workgroups that fill the device (two per CU)
   double c_ref_ = 0, tau_ = 0, y_ = 0, anp_ = 0, d_[3] = {1, 1, 1};
   std::vector<long long> psrc_idx_h_;
   std::vector<double> psrc_sig_h_, psrc_fac_h_;
