@@ -196,6 +196,9 @@ class ElasticSim {
       int v = 0;
       HIPC(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_));
       lds_max_ = v > 0 ? (size_t)v : 65536;
+      int cus = 0;
+      HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_));
+      fill_wgs_ = 2LL * std::max(cus, 1);   // two workgroups per CU
     }
     nvox_ = N_[0] * N_[1] * N_[2];
     dt_ = d.dt;
@@ -258,7 +261,7 @@ class ElasticSim {
   }
 
   PassCfg line_cfg(long long n, bool contig, long long lines, long long outer) {
-    return line_pass_cfg(n, contig, lines, outer, sizeof(Cx<T>), lds_max_);
+    return line_pass_cfg(n, contig, lines, outer, sizeof(Cx<T>), lds_max_, fill_wgs_);
   }
 
   ElArgs<T> base_args(int axis) {
@@ -524,6 +527,7 @@ class ElasticSim {
   double dt_ = 0;
   bool finalized_ = false, cartesian_ = false;
   size_t lds_max_ = 65536;
+  long long fill_wgs_ = 512;
   DevBuf v_[3], sn_[3], ss_[3], scr_[2], p_;
   DevBuf lam_, mu_, rho_;
   double lam_s_ = 0, mu_s_ = 0, rho_s_ = 1, kp_ = 1, ks_ = 1;

@@ -176,8 +176,10 @@ struct PassCfg {
 // line pairs up to 512 threads on strided lines, halved until the lines fit
 // the workgroup's LDS and to the lines that exist.  These passes (elastic,
 // operator tests) use plan family 2 on contiguous lines.
+// fill_wgs > 0: on small grids, fewer lines per workgroup until the launch has
+// fill_wgs workgroups (keeping one full wave per workgroup).
 inline PassCfg line_pass_cfg(long long n, bool contig, long long lines, long long outer, size_t cx_bytes,
-                             size_t lds_max) {
+                             size_t lds_max, long long fill_wgs = 0) {
   PassCfg c;
   c.n = (int)n;
   const int tpl = kw::plan_tpl((int)n, contig ? 2 : 0);
@@ -186,6 +188,11 @@ inline PassCfg line_pass_cfg(long long n, bool contig, long long lines, long lon
   while (lpw > 1 && (size_t)lpw * line_bytes > lds_max) lpw /= 2;
   const long long pairs = (lines + 1) / 2;
   while (lpw > 1 && lpw / 2 >= pairs) lpw /= 2;
+  {
+    const int min_lpw = std::max(1, 64 / tpl);
+    const long long o = contig ? 1 : std::max<long long>(outer, 1);
+    while (lpw / 2 >= min_lpw && ((pairs + lpw - 1) / lpw) * o < fill_wgs) lpw /= 2;
+  }
   c.lpw = c.req = lpw;
   c.block = dim3(lpw * tpl);
   c.shmem = (size_t)lpw * line_bytes;
