@@ -9,9 +9,11 @@ timed region; the timed region is K steps between two device syncs.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 
-Prints ONE JSON line (metric, value = voxel-updates/s, roofline of the
-dominant kernel measured with HIP events on the simulation stream, and the
-oracle timed on the host cores as the CPU baseline).
+Prints ONE JSON line (metric, value = voxel-updates/s, the step roofline on
+64 B per voxel-update, the roofline of the dominant kernel -- the largest
+share of the step -- measured with HIP events on the simulation stream, each
+kernel's figures, the FP32-vs-FP64 rel-L2 over the workload's full 1200
+steps, and the oracle timed on the host cores as the CPU baseline).
 """
 from __future__ import annotations
 
@@ -48,18 +50,47 @@ def kernel_bytes_per_voxel(name: str, nx: int, s: int = 4) -> float:
     }.get(name, 0.0)
 
 
-def measured_traffic(kernel: str, n: int):
-    """HBM bytes per launch of `kernel` on the n^3 workload from the committed rocprofv3
-    PMC summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration),
-    or None when no summary covers it."""
-    path = os.path.join(ROOT, "profiles", f"pmc_{kernel}_{n}cube.md")
+# Kernel functions of one 3-D step (as rocprofv3 names them) and the bench.py pass
+# classes each one runs as: the strided directional pass is one kernel launched twice
+# per step (y and z lines).
+KERNELS = {
+    "k_dir_pass<float,256,false> (strided y/z pass)": ("k_dir_pass_y", "k_dir_pass_z"),
+    "k_dir_pass<float,256,true> (x pass + EOS r2c)": ("k_dir_pass_x_eos",),
+    "k_spec_pass<float,256,0> (spectral y forward)": ("k_spec_pass_y_fwd",),
+    "k_spec_pass<float,256,2> (spectral z fwd*M*inv)": ("k_spec_pass_z_mul",),
+    "k_spec_pass<float,256,1> (spectral y inverse)": ("k_spec_pass_y_inv",),
+    "k_c2r_x<float,256> (c2r x pass)": ("k_c2r_x",),
+}
+
+
+def measured_traffic(kernel_class: str, n: int):
+    """HBM bytes per launch of a pass class on the n^3 workload from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel
+    calibration), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", f"pmc_pstd_step_{n}cube_fp32.md")
     if not os.path.exists(path):
         return None
+    key = f"traffic_bytes_per_launch[{kernel_class}]:"
     with open(path) as f:
         for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key):
                 return float(line.split(":", 1)[1])
     return None
+
+
+def host_cpu():
+    """CPU model and the CPUs this process may use (the CPU baseline's host)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    visible = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return model, visible, os.cpu_count()
 
 
 def cpu_baseline(n_steps_max: int = 20, budget_s: float = 20.0):
@@ -68,8 +99,9 @@ def cpu_baseline(n_steps_max: int = 20, budget_s: float = 20.0):
     from oracle import kwave_oracle as O
     from kwave_amd.configs import ellipsoids_3d
 
-    workers = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    workers = min(workers, 16)
+    # the GPU host allots 16 CPUs to each GPU: the FFT thread pool is sized to that share
+    model, visible, total = host_cpu()
+    workers = min(visible, 16)
     O.set_fft_workers(workers)
     kg, med, src, sen, kw = ellipsoids_3d(Nt=1200)
     stamps = []
@@ -95,7 +127,8 @@ def cpu_baseline(n_steps_max: int = 20, budget_s: float = 20.0):
     return {"value": vox * n / dt, "unit": "voxel-updates/s", "cores": workers, "kind": "port",
             "sample": f"oracle FP32 restatement of the 256^3 workload: 1 warm-up step + {n} timed steps "
                       f"({dt:.1f} s; scipy.fft with {workers} threads, NumPy elementwise on 1 thread), "
-                      f"setup {stamps[0] - t_start:.1f} s excluded"}
+                      f"setup {stamps[0] - t_start:.1f} s excluded",
+            "cpu_model": model, "cpus_visible": visible, "cpus_in_host": total}
 
 
 def bench_batch(a):
@@ -275,8 +308,8 @@ def cpu_baseline_elastic(n: int = 128, n_steps_max: int = 12, budget_s: float = 
     from oracle import kwave_oracle as O
     from kwave_amd.configs import two_layer_elastic_3d
 
-    workers = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
-    workers = min(workers, 16)
+    model, visible, total = host_cpu()
+    workers = min(visible, 16)
     O.set_fft_workers(workers)
     kg, med, src, sen, kw = two_layer_elastic_3d(N=n, Nt=200)
     stamps = []
@@ -297,7 +330,8 @@ def cpu_baseline_elastic(n: int = 128, n_steps_max: int = 12, budget_s: float = 
     dt = stamps[-1] - stamps[0]
     return {"value": n ** 3 * k / dt, "unit": "voxel-updates/s", "cores": workers, "kind": "port",
             "sample": f"oracle FP32 elastic restatement on the {n}^3 cut of the workload: 1 warm-up step + {k} "
-                      f"timed steps ({dt:.1f} s; scipy.fft with {workers} threads)"}
+                      f"timed steps ({dt:.1f} s; scipy.fft with {workers} threads)",
+            "cpu_model": model, "cpus_visible": visible, "cpus_in_host": total}
 
 
 def bench_elastic(a):
@@ -372,6 +406,8 @@ def main():
     ap.add_argument("--N", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 accuracy run")
+    ap.add_argument("--accuracy-nt", type=int, default=1200,
+                    help="steps of the FP32-vs-FP64 accuracy runs (the workload's Nt)")
     ap.add_argument("--config", choices=["c2", "batch", "adjoint", "c4", "elastic"], default="c2")
     ap.add_argument("--firings", type=int, default=8)
     ap.add_argument("--fd-pixels", type=int, default=32)
@@ -424,39 +460,41 @@ def main():
     pmax = sim.sensor("p_max")
     if not np.all(np.isfinite(pmax)):
         raise RuntimeError("non-finite sensor data")
+    sim.close()
+
     fp64_check = None
-    if not a.no_fp64 and sim.steps_done() == nt:
-        # the same kernels compiled in FP64 (parity-tested against the FP64 oracle) on the
-        # full workload, untimed: rel-L2 of the FP32 outputs against them
-        out32 = {f: sim.sensor(f) for f in ("p", "p_max", "p_rms")}
-        sim.close()
-        d64, k64, _ = K.build_desc(kg, med, src, sen, pml_size=kw["pml_size"], pml_alpha=kw["pml_alpha"],
-                                   smooth_p0=False, data_cast=np.float64)
-        with K.Simulation(d64, k64) as s64:
-            s64.run(nt)
-            s64.synchronize()
-            fp64_check = {}
-            for f, v in out32.items():
-                r = s64.sensor(f)
-                fp64_check[f] = float(np.linalg.norm(v - r) / np.linalg.norm(r))
-        fp64_check["steps"] = nt
-    else:
-        sim.close()
+    if not a.no_fp64:
+        fp64_check = accuracy_check(a.N, a.accuracy_nt)
 
     N = a.N
     vox = N ** 3
     secs = t2 - t1
     value = vox * a.steps / secs
-    # dominant kernel = largest share of the step time
-    roof = None
-    if prof:
-        dname, dstat = max(prof.items(), key=lambda kv: kv[1]["ms"])
-        bpv = kernel_bytes_per_voxel(dname, N)
-        achieved = bpv * vox / (dstat["ms"] * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": dname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dname, N),
-                "bytes_per_launch": bpv * vox, "avg_launch_ms": dstat["ms"]}
     step_b = ellipsoids_3d_bytes_per_voxel_update(4)
+    ms_step = secs * 1e3 / a.steps
+    # per kernel function: launches per step, mean launch time (HIP events on the sim
+    # stream), algorithmic bytes per launch and the rate they imply
+    kernels = {}
+    for kname, classes in KERNELS.items():
+        got = [prof[c]["ms"] for c in classes if c in prof]
+        if not got:
+            continue
+        bpl = sum(kernel_bytes_per_voxel(c, N) for c in classes) / len(classes) * vox
+        avg = sum(got) / len(got)
+        kernels[kname] = {"launches_per_step": len(got), "avg_launch_ms": avg, "bytes_per_launch": bpl,
+                          "GB_s": bpl / (avg * 1e-3) / 1e9, "share_of_step": len(got) * avg / ms_step,
+                          "traffic_bytes_per_launch": None}
+        t = [measured_traffic(c, N) for c in classes]
+        if all(x is not None for x in t):
+            kernels[kname]["traffic_bytes_per_launch"] = sum(t) / len(t)
+    roof = None
+    if kernels:
+        # dominant kernel = largest share of the step time
+        dname, d = max(kernels.items(), key=lambda kv: kv[1]["share_of_step"])
+        roof = {"bound": "hbm", "kernel": dname, "achieved": d["GB_s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": d["GB_s"] / HBM_PEAK_GBS, "traffic": d["traffic_bytes_per_launch"],
+                "bytes_per_launch": d["bytes_per_launch"], "avg_launch_ms": d["avg_launch_ms"],
+                "share_of_step": d["share_of_step"]}
     out = {
         "metric": METRIC,
         "value": value,
@@ -464,7 +502,7 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": secs * 1e3 / a.steps,
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -477,7 +515,7 @@ def main():
         "roofline": roof,
         "step_roofline": {"bytes_per_voxel_update": step_b, "achieved": step_b * value / 1e9,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_b * value / 1e9 / HBM_PEAK_GBS},
-        "kernels_ms": {k: round(v["ms"], 5) for k, v in prof.items()},
+        "kernels": kernels,
         "fp32_vs_fp64_rel_l2": fp64_check,
         "setup_s": setup_s,
     }
@@ -487,6 +525,32 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+def accuracy_check(N: int, nt: int):
+    """FP32 kernels against the same kernels compiled in FP64 (parity-tested against the
+    FP64 oracle) on the full workload: rel-L2 of p, p_max and p_rms over all `nt` steps
+    (the workload's Nt = 1200 by default, so the pulse launched at z = 21 has crossed the
+    z = 200 sensor plane), untimed."""
+    import numpy as np
+    import kwave_amd as K
+    from kwave_amd.configs import ellipsoids_3d
+
+    print(f"[bench] accuracy: FP32 vs FP64 over {nt} steps", file=sys.stderr, flush=True)
+    kg, med, src, sen, kw = ellipsoids_3d(N=N, Nt=nt)
+    outs = {}
+    for dc in (np.float32, np.float64):
+        d, k, _ = K.build_desc(kg, med, src, sen, pml_size=kw["pml_size"], pml_alpha=kw["pml_alpha"],
+                               smooth_p0=False, data_cast=dc)
+        with K.Simulation(d, k) as s:
+            s.run(nt)
+            s.synchronize()
+            outs[dc] = {f: s.sensor(f) for f in ("p", "p_max", "p_rms")}
+    res = {f: float(np.linalg.norm(outs[np.float32][f] - outs[np.float64][f]) / np.linalg.norm(outs[np.float64][f]))
+           for f in ("p", "p_max", "p_rms")}
+    res["steps"] = nt
+    res["sensor_peak_pa"] = float(np.abs(outs[np.float64]["p_max"]).max())
+    return res
 
 
 if __name__ == "__main__":

@@ -158,6 +158,17 @@ int32_t kw_ctx_destroy(kw_ctx* ctx);
 int32_t kw_fft_length_supported(int64_t n);
 
 /*
+ * 1 if a grid of ndim axes N[0..ndim-1] runs in `precision` (4 or 8), else 0:
+ * every axis length has a line-FFT plan and no strided line spans more than
+ * 4 GiB (the line passes use 32-bit per-lane byte offsets from a 64-bit
+ * workgroup base; e.g. 1024^3 in Float32 and 1024^2 x 512 in Float64 exceed
+ * it).  kw_sim_create / kw_elastic_create reject such grids with
+ * KW_ERR_UNSUPPORTED before allocating.  Host-only; no device is touched.
+ * No reference counterpart (FFTW plans any size that fits in memory).
+ */
+int32_t kw_grid_supported(int32_t ndim, const int64_t* N, int32_t precision);
+
+/*
  * Build a simulation: host-side FP64 tables (k vectors, staggered shifts,
  * kappa, absorption, PML, twiddles), device buffers, p0 initialisation
  * (initialize_p0_*, first_order_steps.jl:58-191).  Replaces

@@ -45,8 +45,32 @@ FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-I" + HERE,
          "-I" + os.path.join(os.path.dirname(os.path.dirname(HERE)), "include")] + DEVICE_FLAGS
 
 
-def _compile(src: str) -> str:
+def _deps(path: str, seen=None) -> set:
+    """The file and every quoted #include it reaches (recursively)."""
+    seen = set() if seen is None else seen
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for line in f:
+            s = line.strip()
+            if s.startswith("#include \""):
+                inc = s.split("\"")[1]
+                _deps(os.path.normpath(os.path.join(os.path.dirname(path), inc)), seen)
+    return seen
+
+
+def _stale(src: str, obj: str) -> bool:
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in _deps(os.path.join(HERE, src)))
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+    if incremental and not _stale(src, obj):
+        return obj
     cmd = [HIPCC] + FLAGS + ["-c", os.path.join(HERE, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -55,12 +79,14 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 0, verbose: bool = True) -> str:
+def build(jobs: int = 0, verbose: bool = True, incremental: bool = False) -> str:
+    """Compile every translation unit (incremental: only those whose source or
+    included headers changed since their object was built) and link."""
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(len(SOURCES), os.cpu_count() or 4)
     t0 = time.time()
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda s: _compile(s, incremental), SOURCES))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -75,9 +101,11 @@ def build(jobs: int = 0, verbose: bool = True) -> str:
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", "--jobs", type=int, default=0)
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only translation units whose sources or headers changed")
     a = ap.parse_args()
     try:
-        build(a.jobs)
+        build(a.jobs, incremental=a.incremental)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

@@ -523,6 +523,7 @@ class Sim final : public SimBase {
     }
     if (d.dt == 0.0 || d.Nt == 0)
       raise(KW_ERR_INVALID, "Time array not set. Call make_time!(kgrid, sound_speed) before running.");
+    check_lane_offsets(nd_, N_, (int)sizeof(T));
     if (d.map_storage != KW_MAPS_NATIVE && d.map_storage != KW_MAPS_FP16)
       raise(KW_ERR_INVALID, "unknown map storage mode");
     maps_f16_ = d.map_storage == KW_MAPS_FP16;
@@ -1668,6 +1669,16 @@ int32_t kw_ctx_destroy(kw_ctx* ctx) {
 }
 
 int32_t kw_fft_length_supported(int64_t n) { return size_supported(n) ? 1 : 0; }
+
+int32_t kw_grid_supported(int32_t ndim, const int64_t* N, int32_t precision) {
+  if (ndim < 1 || ndim > 3 || N == nullptr || (precision != 4 && precision != 8)) return 0;
+  long long n[3] = {1, 1, 1};
+  for (int a = 0; a < ndim; ++a) {
+    if (!size_supported(N[a])) return 0;
+    n[a] = N[a];
+  }
+  return lane_offsets_fit(ndim, n, precision) ? 1 : 0;
+}
 
 int32_t kw_sim_create(kw_ctx* ctx, const kw_sim_desc* desc, kw_sim** out) {
   KW_GUARD_BEGIN

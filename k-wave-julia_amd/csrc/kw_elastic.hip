@@ -200,6 +200,7 @@ class ElasticSim {
       HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_));
       fill_wgs_ = 2LL * std::max(cus, 1);   // two workgroups per CU
     }
+    check_lane_offsets(nd_, N_, (int)sizeof(T));
     nvox_ = N_[0] * N_[1] * N_[2];
     dt_ = d.dt;
     Nt_ = d.Nt;

@@ -203,6 +203,44 @@ inline PassCfg line_pass_cfg(long long n, bool contig, long long lines, long lon
   return c;
 }
 
+// Line passes address a uniform 64-bit workgroup base plus 32-bit unsigned
+// per-lane byte offsets (scalar-base + vector-offset loads).  The largest
+// lane offset is the span of one strided line: stride x (length - 1) plus the
+// lines of one workgroup (< 1024 elements), in the element size of the
+// buffer the pass walks.  Strided field passes walk T elements with strides
+// Nx (y lines) and Nx*Ny (z lines); spectral passes walk complex elements of
+// the half spectrum with strides NHP (y) and NHP*Ny (z), NHP = Nx/2+1 rounded
+// up to 16; the p0 smoothing pass walks complex copies of the full field
+// with strides Nx and Nx*Ny.  Contiguous x-line passes stay below
+// 1024 * Nx complex elements.
+inline unsigned long long max_lane_offset_bytes(int nd, const long long N[3], int prec) {
+  const unsigned long long s = (unsigned long long)prec, cx = 2 * s;
+  const unsigned long long nx = (unsigned long long)N[0], ny = nd > 1 ? (unsigned long long)N[1] : 1ULL,
+                           nz = nd > 2 ? (unsigned long long)N[2] : 1ULL;
+  const unsigned long long nhp = (nx / 2 + 1 + 15) / 16 * 16, w = 1024;
+  unsigned long long m = (w * nx) * cx;
+  auto upd = [&](unsigned long long stride, unsigned long long len, unsigned long long elem) {
+    if (len > 1) m = std::max(m, (stride * (len - 1) + w) * elem);
+  };
+  upd(nx, ny, cx);           // y lines: fields (T) and smoothing (complex)
+  upd(nx * ny, nz, cx);      // z lines: fields (T) and smoothing (complex)
+  upd(nhp, ny, cx);          // spectral y
+  upd(nhp * ny, nz, cx);     // spectral z
+  return m;
+}
+// Grids whose lane offsets fit 32 bits (every pass, both precisions' element
+// sizes as given); larger grids are rejected before any allocation.
+inline bool lane_offsets_fit(int nd, const long long N[3], int prec) {
+  return max_lane_offset_bytes(nd, N, prec) <= 0xFFFFFFFFULL;
+}
+inline void check_lane_offsets(int nd, const long long N[3], int prec) {
+  if (!lane_offsets_fit(nd, N, prec))
+    raise(KW_ERR_UNSUPPORTED, "grid " + std::to_string(N[0]) + "x" + std::to_string(nd > 1 ? N[1] : 1) + "x" +
+                                  std::to_string(nd > 2 ? N[2] : 1) + " in " + (prec == 8 ? "Float64" : "Float32") +
+                                  ": a strided line spans more than 4 GiB, beyond the 32-bit lane offsets of the "
+                                  "line passes");
+}
+
 // EOS multipliers on the octant of |k| (first_order.jl:13-42, 361-375,
 // 581-595; first_order_steps.jl:245-289), FP64, without the 1/N of the
 // inverse transform: m1 = c^2 kappa (1 + tau |k|^y), m2 = -c^2 kappa eta |k|^(y-1)

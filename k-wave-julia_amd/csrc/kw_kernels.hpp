@@ -909,7 +909,7 @@ __device__ __forceinline__ void record_body(const RecArgs<T>& a, int j, int t_in
   if (a.rec & REC_P) a.p_ts[o] = pv;
   if (a.rec & REC_P_MAX) a.p_max[j] = a.p_max[j] > pv ? a.p_max[j] : pv;
   if (a.rec & REC_P_MIN) a.p_min[j] = a.p_min[j] < pv ? a.p_min[j] : pv;
-  if (a.rec & REC_P_RMS) a.p_rms[j] += (double)(pv * pv);
+  if (a.rec & REC_P_RMS) a.p_rms[j] += (double)pv * (double)pv;
   if ((a.rec & REC_P_FINAL) && t_index == a.Nt) a.p_final[j] = pv;
   T uv[3] = {0, 0, 0};
   // velocities only when a velocity or intensity field is recorded
@@ -920,7 +920,7 @@ __device__ __forceinline__ void record_body(const RecArgs<T>& a, int j, int t_in
   if ((a.rec & REC_UZ) && a.ndim > 2) a.u_ts[2][o] = uv[2];
   if (a.rec & (REC_U_MAX | REC_U_RMS | REC_U_FINAL)) {
     double m2 = 0.0;
-    for (int d = 0; d < a.ndim; ++d) m2 += (double)(uv[d] * uv[d]);
+    for (int d = 0; d < a.ndim; ++d) m2 += (double)uv[d] * (double)uv[d];
     const double mag = sqrt(m2);
     if (a.rec & REC_U_MAX) a.u_max[j] = a.u_max[j] > mag ? a.u_max[j] : mag;
     if (a.rec & REC_U_RMS) a.u_rms[j] += mag * mag;

@@ -136,6 +136,11 @@ int32_t kw_test_spectral_gradient(kw_ctx* ctx, int32_t precision, int32_t ndim, 
   for (int a = 0; a < ndim; ++a)
     if (N[a] < 1) raise(KW_ERR_INVALID, "grid size must be positive");
   if (!size_supported(N[axis])) raise(KW_ERR_UNSUPPORTED, "axis length has no line-FFT plan");
+  if (precision == 4 || precision == 8) {
+    long long n3[3] = {1, 1, 1};
+    for (int a = 0; a < ndim; ++a) n3[a] = N[a];
+    check_lane_offsets(ndim, n3, precision);
+  }
   if (sign == 0 && axis != 0 && N[axis] % 2 == 0)
     raise(KW_ERR_UNSUPPORTED,
           "an unstaggered derivative along y/z of an even axis couples the Nyquist row across x in the N-D "

@@ -31,7 +31,7 @@ ALPHA_MODE = {"no_absorption": 0, "no_dispersion": 1, "stokes": 2}
 # every symbol include/kwave_hip.h declares
 EXPORTS = (
     "kw_abi_version", "kw_last_error", "kw_device_count", "kw_ctx_create", "kw_ctx_destroy",
-    "kw_fft_length_supported", "kw_sim_create", "kw_sim_run", "kw_sim_synchronize",
+    "kw_fft_length_supported", "kw_grid_supported", "kw_sim_create", "kw_sim_run", "kw_sim_synchronize",
     "kw_sim_steps_done", "kw_sim_get_field", "kw_sim_sensor_shape", "kw_sim_get_sensor",
     "kw_sim_profile", "kw_sim_grad_c0", "kw_sim_destroy", "kw_autotune", "kw_tune_export", "kw_tune_import",
     "kw_tune_clear", "kw_batch_create", "kw_batch_run", "kw_batch_synchronize",
@@ -162,6 +162,7 @@ def load():
             "kw_ctx_create": ([C.c_int32, C.POINTER(P)], C.c_int32),
             "kw_ctx_destroy": ([P], C.c_int32),
             "kw_fft_length_supported": ([C.c_int64], C.c_int32),
+            "kw_grid_supported": ([C.c_int32, C.POINTER(C.c_int64), C.c_int32], C.c_int32),
             "kw_sim_create": ([P, C.POINTER(SimDesc), C.POINTER(P)], C.c_int32),
             "kw_sim_run": ([P, C.c_int64], C.c_int32),
             "kw_sim_synchronize": ([P], C.c_int32),

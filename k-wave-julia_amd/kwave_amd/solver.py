@@ -305,6 +305,9 @@ def build_desc(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSourc
             info["n_sensor"] = m.shape[1]
             info["cartesian"] = True
     if tr:
+        # first_order.jl:455 reads tr_data[j, tr_t] for every sensor point j: too few rows is a BoundsError
+        if trd.ndim != 2 or trd.shape[0] < info["n_sensor"]:
+            raise IndexError("time_reversal_boundary_data has fewer rows than the sensor mask has points")
         trc = np.asfortranarray(trd)
         keep.append(trc)
         d.tr_data = trc.ctypes.data_as(_DP)

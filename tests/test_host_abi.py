@@ -37,6 +37,25 @@ def test_library_exports_every_declared_symbol():
         assert lib.kw_fft_length_supported(n) == 0
 
 
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libkwave_hip.so not built")
+def test_grid_lane_offset_limit():
+    """Strided lines spanning more than 4 GiB are rejected (32-bit lane offsets)."""
+    lib = _lib.load()
+
+    def ok(shape, prec):
+        n = (C.c_int64 * 3)(*(list(shape) + [1] * (3 - len(shape))))
+        return lib.kw_grid_supported(len(shape), n, prec)
+
+    assert ok((256, 256, 256), 4) == 1 and ok((256, 256, 256), 8) == 1
+    assert ok((512, 512, 512), 4) == 1 and ok((512, 512, 512), 8) == 1
+    assert ok((1024, 1024, 1024), 4) == 0          # spectral z line: 528*1024*1023 complex = 4.4e9 B
+    assert ok((1024, 1024, 512), 8) == 0           # 528*1024*511 complex128 = 4.4e9 B
+    assert ok((2048, 2048, 256), 4) == 0
+    assert ok((4096, 4096), 8) == 1                # 2-D lines stay far below the limit
+    assert ok((1000, 1000), 4) == 0                # no line-FFT plan
+    assert ok((256, 256, 256), 2) == 0             # unknown precision
+
+
 def test_ctypes_descriptor_layout_matches_header():
     """Compile the header with gcc and compare sizeof/offsetof with ctypes."""
     fields = [f[0] for f in _lib.SimDesc._fields_]
@@ -148,3 +167,17 @@ def test_errors_from_every_translation_unit_reach_kw_last_error():
     lib = _lib.load()
     assert lib.kw_elastic_create(None, None, None) == _lib.KW_ERR_INVALID
     assert b"null argument" in lib.kw_last_error()
+
+
+def test_time_reversal_data_rows_checked():
+    """Fewer TR-data rows than sensor points is the reference's BoundsError (first_order.jl:455)."""
+    kg, med, _, m = _small()
+    K.make_time(kg, 1500.0)
+    m = m.copy()
+    m[10, 10] = True                                     # two sensor points
+    sen = K.KWaveSensor(mask=m, time_reversal_boundary_data=np.zeros((1, 40)))
+    with pytest.raises(IndexError, match="fewer rows"):
+        K.build_desc(kg, med, K.KWaveSource(), sen)
+    sen = K.KWaveSensor(mask=m, time_reversal_boundary_data=np.zeros((2, 40)))
+    d, keep, info = K.build_desc(kg, med, K.KWaveSource(), sen)
+    assert info["tr"] and d.Nt == 40 and d.n_sensor == 2
