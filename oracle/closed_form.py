@@ -22,8 +22,15 @@ where prev^1 = 0 and prev^n = rho^{n-1} (first_order.jl:638-639,
 first_order_steps.jl:438-440), kappa = sin(c|k|dt/2)/(c|k|dt/2)
 (first_order.jl:35-42) and S^n is the density injected at step n by an
 additive pressure source, p_src/c^2 summed over the split components
-(first_order_steps.jl:619-640).  The start is rho^0 = 0, rho^1 = S^1 for a
-source-driven run, and rho^0 = p0/c^2, p^0 = p0, u^0 = -dt/(2 rho0) D+ p0,
+(first_order_steps.jl:619-640).  An additive velocity source V_d^n (the raw
+values added to u_d after its update at step n, first_order_steps.jl:404-415,
+541-559, 586-617, 646-690) enters through the density update of the same
+step, rho_d^n = rho_d^{n-1} - dt rho0 D-_d u_d^n, so the recurrence gains
+
+    - dt rho0 sum_d D-_d V_d^{n+1}
+
+on its right-hand side (and rho^1 gains - dt rho0 sum_d D-_d V_d^1).  The
+start is rho^0 = 0, rho^1 = S^1 for a source-driven run, and rho^0 = p0/c^2, p^0 = p0, u^0 = -dt/(2 rho0) D+ p0,
 giving rho^1 = p0/c^2 - 1.5 dt^2 |k|^2 p0 + S^1 for an initial-value run
 (first_order_steps.jl:97-137; smooth_p0 = false).
 
@@ -57,12 +64,15 @@ def recurrence_trace(shape: Sequence[int], d: Sequence[float], c0: float, rho0: 
                      dt: float, Nt: int, sensor_idx: np.ndarray,
                      src_idx: Optional[np.ndarray] = None, src_signal: Optional[np.ndarray] = None,
                      p0: Optional[np.ndarray] = None, alpha: Optional[dict] = None,
-                     c_ref: Optional[float] = None):
+                     c_ref: Optional[float] = None, usrc_idx: Optional[np.ndarray] = None,
+                     usrc_signals: Optional[Sequence[Optional[np.ndarray]]] = None):
     """Sensor pressure traces (n_sensor x Nt) and the final pressure field.
 
     ``src_idx`` are column-major linear indices, ``src_signal`` is
     (n_src x ncols) sampled at min(t, ncols) (first_order_steps.jl:625).
     ``alpha`` = dict(tau, eta, y, mode) overrides the absorption prefactors.
+    ``usrc_idx`` / ``usrc_signals`` (one (n x ncols) array or None per axis)
+    describe an additive velocity source.
     """
     shape = tuple(int(n) for n in shape)
     nd = len(shape)
@@ -98,16 +108,38 @@ def recurrence_trace(shape: Sequence[int], d: Sequence[float], c0: float, rho0: 
         np.add.at(f, src_idx, src_signal[:, col] / c2)
         return np.fft.fftn(f.reshape(shape, order="F"))
 
+    # staggered divergence multipliers D-_d = i k_d exp(-i k_d h/2) on the full grid
+    dminus = []
+    for a, (n, h) in enumerate(zip(shape, d)):
+        kv = _kvec(n, h)
+        sh = [1] * nd
+        sh[a] = n
+        dminus.append((1j * kv * np.exp(-1j * kv * h / 2)).reshape(sh))
+
+    def vel_hat(t):
+        """- dt rho0 sum_d D-_d FFT(V_d^t), the density forcing of the velocity source at step t."""
+        if usrc_idx is None or usrc_signals is None:
+            return 0.0
+        acc = np.zeros(shape, dtype=complex)
+        for a, sig in enumerate(usrc_signals):
+            if sig is None:
+                continue
+            col = min(t, sig.shape[1]) - 1
+            f = np.zeros(int(np.prod(shape)))
+            np.add.at(f, usrc_idx, sig[:, col])
+            acc += dminus[a] * np.fft.fftn(f.reshape(shape, order="F"))
+        return -dt * rho0 * acc
+
     sens = np.asarray(sensor_idx)
     out = np.zeros((sens.size, Nt))
     zero = np.zeros(shape, dtype=complex)
     if p0 is not None:
         P0 = np.fft.fftn(np.asarray(p0, dtype=np.float64))
         rho_prev_state = P0 / c2                                   # rho^0
-        rho_n = P0 / c2 - 1.5 * dt * dt * k2 * P0 + src_hat(1)     # rho^1
+        rho_n = P0 / c2 - 1.5 * dt * dt * k2 * P0 + src_hat(1) + vel_hat(1)   # rho^1
     else:
         rho_prev_state = zero
-        rho_n = src_hat(1) + zero
+        rho_n = src_hat(1) + vel_hat(1) + zero
     S_prev = src_hat(1)
     eos_prev = zero                                                # prev^1 = 0
     p_hat = None
@@ -118,7 +150,7 @@ def recurrence_trace(shape: Sequence[int], d: Sequence[float], c0: float, rho0: 
         if t == Nt:
             break
         S_next = src_hat(t + 1)
-        rho_next = 2 * rho_n - rho_prev_state - dt * dt * k2 * p_hat + (S_next - S_prev)
+        rho_next = 2 * rho_n - rho_prev_state - dt * dt * k2 * p_hat + (S_next - S_prev) + vel_hat(t + 1)
         rho_prev_state, eos_prev = rho_n, rho_n
         rho_n, S_prev = rho_next, S_next
     return out, np.fft.ifftn(p_hat).real

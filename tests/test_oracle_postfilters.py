@@ -14,6 +14,8 @@ directivity argument without the 1/2) are rejected.
 """
 import math
 
+import pytest
+
 import numpy as np
 
 from oracle import kwave_oracle as O
@@ -76,3 +78,49 @@ def test_directivity_factor():
     assert np.allclose(dc["p"], 1.0, atol=1e-12)                          # DC passes
     # planted mistake: no 1/2 in the aperture phase
     assert np.abs(math.sin(2 * x) / (2 * x) * rows[2] - out[2]).max() > 1e-3
+
+
+def test_frequency_response_law_at_every_bin():
+    """Every DFT bin is an eigenvector (a tone keeps its phase) and log(gain) is one
+    quadratic in |f| with its vertex at fc and curvature -1/bw^2: the Gaussian law
+    holds across the whole band, not only at fc and fc +- bw."""
+    Nt, dt = 96, 1e-8
+    g = O.Grid((16,), (1e-4,), dt, Nt)
+    df = 1.0 / (Nt * dt)
+    fc, bw = 13.5 * df, 6.25 * df                # centre between bins
+    ms = np.arange(0, Nt // 2)
+    rows = np.stack([_tone(Nt, m, 0.1 * m) for m in ms])
+    data = {"p": rows.copy()}
+    O.apply_frequency_response(data, _Sensor(frequency_response=(fc, bw)), g)
+    gains = []
+    for m, r, o in zip(ms, rows, data["p"]):
+        gm = float(np.dot(o, r) / np.dot(r, r))
+        assert np.allclose(o, gm * r, atol=1e-13), m             # eigenvector, phase kept
+        gains.append(gm)
+    lg = np.log(np.asarray(gains))
+    f = ms * df
+    # log G = -(f - fc)^2 / (2 bw^2): second difference constant, first difference zero at fc
+    second = lg[2:] - 2 * lg[1:-1] + lg[:-2]
+    assert np.allclose(second, -(df / bw) ** 2, rtol=0, atol=1e-9)
+    first = np.diff(lg) / df                                     # slope at the midpoints
+    mid = (f[1:] + f[:-1]) / 2
+    assert np.allclose(first, -(mid - fc) / bw ** 2, rtol=1e-9, atol=1e-18)
+    assert lg.max() <= 0 and np.argmax(lg) in (13, 14)
+
+
+def test_directivity_law_at_every_bin():
+    """Every bin is scaled by sin(c m)/(c m) with one constant c per sensor (the
+    aperture phase grows linearly with |f|), and the sensor with angle 0 is untouched."""
+    Nt, dt = 80, 2e-8
+    g = O.Grid((30, 30), (1e-4, 1e-4), dt, Nt)
+    df = 1.0 / (Nt * dt)
+    size, ang = 4e-4, 0.3
+    c = (2 * math.pi * df / 1e-4) * size * math.sin(ang) / 2      # aperture phase per bin
+    for m in range(1, Nt // 2):
+        data = {"p": np.stack([_tone(Nt, m, 0.2), _tone(Nt, m, 0.2)])}
+        ref = data["p"].copy()
+        O.apply_sensor_directivity(data, _Sensor(directivity_angle=np.array([ang, 0.0]), directivity_size=size), g)
+        gm = float(np.dot(data["p"][0], ref[0]) / np.dot(ref[0], ref[0]))
+        assert np.allclose(data["p"][0], gm * ref[0], atol=1e-13)
+        assert gm == pytest.approx(math.sin(c * m) / (c * m), rel=1e-10, abs=1e-13)
+        assert np.array_equal(data["p"][1], ref[1])
