@@ -206,7 +206,9 @@ int32_t kw_sim_profile(kw_sim* sim, int64_t n_steps, int32_t max_classes, int32_
  * Launch-shape autotuner (no counterpart in the reference, whose FFTW plans
  * are tuned by FFTW.MEASURE in create_fft_plans, fft_utils.jl:44-62).  Runs
  * timed trials (steps timed steps each; time reversal ignored) with 1, 1/2,
- * 1/4 and 1/8 of the default lines per workgroup for every pass and keeps
+ * 1/4 and 1/8 of the default lines per workgroup and with twice the default
+ * (up to the pass's 512-thread limit, 256 for the complex-to-real pass, and
+ * the workgroup LDS) for every pass and keeps
  * the fastest per (precision, pass, line length) in a process-wide table
  * that later kw_sim_create / kw_batch_create calls read.  With n_sims == 1
  * the trials run descs[0] as a single simulation; with n_sims > 1 they run

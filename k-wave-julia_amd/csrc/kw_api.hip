@@ -30,13 +30,22 @@ namespace {
 enum TuneClass { TC_DIR_X = 0, TC_DIR_Y, TC_DIR_Z, TC_SPEC_Y, TC_SPEC_Z, TC_C2R, TC_COUNT };
 std::mutex g_tune_mu;
 std::map<std::array<int, 3>, int> g_tune;
-thread_local int g_tune_div = 0;  // > 0 inside a trial: lines per workgroup = default / g_tune_div
+// Inside an autotuner trial: lines per workgroup = default >> g_tune_shift
+// (shift -1 doubles it, up to the pass's maximum); kNoTrial outside trials.
+constexpr int kNoTrial = 99;
+thread_local int g_tune_shift = kNoTrial;
 
-int tuned_lpw(int prec, int cls, int n, int dflt) {
-  if (g_tune_div > 0) return std::max(1, dflt / g_tune_div);
+// Lines per workgroup of a pass: the tuning table's entry for (precision,
+// class, line length) or the trial's choice, clamped to [1, maxl] (maxl: the
+// most lines that fit the pass's thread limit and the workgroup LDS).
+int tuned_lpw(int prec, int cls, int n, int dflt, int maxl) {
+  if (g_tune_shift != kNoTrial) {
+    const int v = g_tune_shift >= 0 ? dflt >> g_tune_shift : dflt << (-g_tune_shift);
+    return std::max(1, std::min(maxl, v));
+  }
   std::lock_guard<std::mutex> lk(g_tune_mu);
   auto it = g_tune.find({prec, cls, n});
-  return it == g_tune.end() ? dflt : std::min(dflt, std::max(1, it->second));
+  return it == g_tune.end() ? dflt : std::min(maxl, std::max(1, it->second));
 }
 
 // ---------------------------------------------------------------------------
@@ -622,7 +631,9 @@ class Sim final : public SimBase {
     // when that fits 512 threads; contiguous lines: 256 threads
     int lpw = (contig ? 256 : std::max(256, std::min(512, 16 * tpl))) / tpl;
     while (lpw > 1 && (size_t)lpw * line_bytes > lds_max_) lpw /= 2;
-    lpw = tuned_lpw((int)sizeof(T), cls, (int)n, lpw);
+    int maxl = (cls == TC_C2R ? 256 : 512) / tpl;   // the kernels' launch bounds
+    while (maxl > 1 && (size_t)maxl * line_bytes > lds_max_) maxl /= 2;
+    lpw = tuned_lpw((int)sizeof(T), cls, (int)n, lpw, std::max(maxl, 1));
     c.req = lpw;
     // pairs that exist
     const long long pairs = (lines + 1) / 2;
@@ -654,7 +665,9 @@ class Sim final : public SimBase {
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
     int lpw = std::max(256, std::min(512, 16 * tpl)) / tpl;   // as for strided passes
     while (lpw > 1 && (size_t)lpw * line_bytes > lds_max_) lpw /= 2;
-    lpw = tuned_lpw((int)sizeof(T), cls, (int)n, lpw);
+    int maxl = 512 / tpl;
+    while (maxl > 1 && (size_t)maxl * line_bytes > lds_max_) maxl /= 2;
+    lpw = tuned_lpw((int)sizeof(T), cls, (int)n, lpw, std::max(maxl, 1));
     c.req = lpw;
     while (!batch_mode_ && lpw > 1 && lpw / 2 >= NH_) lpw /= 2;
     if (!batch_mode_) {   // as in pass_cfg: fill the device on small grids
@@ -1556,32 +1569,37 @@ void autotune(int dev, const kw_sim_desc& d0, int steps) {
   double best_ms[TC_COUNT];
   int best_lp[TC_COUNT], best_n[TC_COUNT] = {0};
   for (int c = 0; c < TC_COUNT; ++c) best_ms[c] = INFINITY;
-  int prev_lp[TC_COUNT] = {0};
-  for (int div = 1; div <= 8; div *= 2) {
+  // trials: the default, then halving (while any class still changes), then doubling
+  int prev_lp[TC_COUNT] = {0}, dflt_lp[TC_COUNT] = {0};
+  bool shrinking = true;
+  for (int shift : {0, 1, 2, 3, -1}) {
+    if (shift > 1 && !shrinking) continue;
     std::unique_ptr<Sim<T>> s;
-    g_tune_div = div;
+    g_tune_shift = shift;
     try {
       s.reset(new Sim<T>(dev, d, false));
     } catch (...) {
-      g_tune_div = 0;
+      g_tune_shift = kNoTrial;
       throw;
     }
-    g_tune_div = 0;
+    g_tune_shift = kNoTrial;
     double ms[TC_COUNT];
     int lp[TC_COUNT], nn[TC_COUNT];
     s->trial(warm, steps, ms, lp, nn);
     bool changed = false;
     for (int c = 0; c < TC_COUNT; ++c) {
       if (nn[c] == 0) continue;
-      if (lp[c] != prev_lp[c]) changed = true;
+      const int ref = shift == -1 ? dflt_lp[c] : prev_lp[c];
+      if (lp[c] != ref) changed = true;
       prev_lp[c] = lp[c];
+      if (shift == 0) dflt_lp[c] = lp[c];
       if (ms[c] < best_ms[c]) {
         best_ms[c] = ms[c];
         best_lp[c] = lp[c];
         best_n[c] = nn[c];
       }
     }
-    if (!changed) break;  // every class already at one line per workgroup
+    if (shift > 0 && !changed) shrinking = false;   // every class already at one line per workgroup
   }
   std::lock_guard<std::mutex> lk(g_tune_mu);
   for (int c = 0; c < TC_COUNT; ++c)
@@ -1600,17 +1618,19 @@ void autotune_batch(int dev, const kw_sim_desc* d0, int n, int steps) {
     x.Nt = warm + steps;
   }
   std::map<std::array<int, 2>, std::pair<double, int>> best;
-  std::vector<int> prev;
-  for (int div = 1; div <= 8; div *= 2) {
+  std::vector<int> prev, dflt;
+  // trials: the default, halving while any class still changes, then doubling
+  for (int shift : {0, 1, 2, 3, -1}) {
+    if (shift > 0 && shift != 1 && prev.empty()) continue;
     std::unique_ptr<Batch<T>> b;
-    g_tune_div = div;
+    g_tune_shift = shift;
     try {
       b.reset(new Batch<T>(dev, d.data(), n, false));
     } catch (...) {
-      g_tune_div = 0;
+      g_tune_shift = kNoTrial;
       throw;
     }
-    g_tune_div = 0;
+    g_tune_shift = kNoTrial;
     std::map<std::array<int, 2>, std::pair<double, int>> r;
     b->trial(warm, steps, r);
     std::vector<int> lp;
@@ -1619,8 +1639,9 @@ void autotune_batch(int dev, const kw_sim_desc* d0, int n, int steps) {
       auto it = best.find(kv.first);
       if (it == best.end() || kv.second.first < it->second.first) best[kv.first] = kv.second;
     }
-    if (lp == prev) break;  // no class changed its shape any more
-    prev = lp;
+    if (shift == 0) dflt = lp;
+    if (shift > 0 && lp == prev) prev.clear();   // no class changes any more: skip to doubling
+    else prev = lp;
   }
   std::lock_guard<std::mutex> lk(g_tune_mu);
   for (auto& kv : best) g_tune[{(int)sizeof(T), kv.first[0], kv.first[1]}] = kv.second.second;
