@@ -21,7 +21,8 @@ from .adjoint import gradient_c0, misfit_rrule
 from .elastic import ElasticSimulation, pstd_elastic_2d, pstd_elastic_3d
 from .transducer import (KWaveTransducer, combine_transducer_sensor_data, get_transducer_binary_mask,
                          get_transducer_source)
-from . import tuning
+from .hdf5 import read_matrix, read_output, write_attributes, write_flags, write_grid, write_matrix
+from . import cli, hdf5, tuning
 from ._lib import KWaveHIPError, LIB_PATH
 
 __all__ = [
@@ -36,4 +37,5 @@ __all__ = [
     "KWaveArray", "ArcElement", "BowlElement", "DiscElement", "RectElement", "add_arc_element", "add_bowl_element",
     "add_disc_element", "add_rect_element", "get_element_binary_mask", "get_array_binary_mask",
     "get_distributed_source_signal", "combine_sensor_data", "KWaveHIPError", "LIB_PATH",
+    "write_matrix", "write_grid", "write_flags", "write_attributes", "read_matrix", "read_output", "hdf5", "cli",
 ]

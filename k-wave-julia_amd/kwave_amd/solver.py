@@ -329,6 +329,31 @@ def build_desc(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSourc
     return d, keep, info
 
 
+def _save_to_disk(path: str, kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSource,
+                  sensor: KWaveSensor, pml_size, pml_alpha) -> SimulationOutput:
+    """Save-to-disk mode (first_order.jl:129-140 1-D, 329-345 2-D, 546-562 3-D): after the
+    input validation, write the k-Wave C++ input file (attributes, grid, c0 or c_ref,
+    rho0 for 2-D/3-D maps, p0) and return an empty SimulationOutput without running."""
+    from . import hdf5
+    if kgrid.Nt == 0 or kgrid.dt == 0:
+        raise ValueError("Time array not set. Call make_time!(kgrid, sound_speed) before running.")
+    validate_record_fields(sensor)
+    nd = kgrid.ndim
+    sizes = tuple(pml_size) if isinstance(pml_size, (tuple, list)) else (pml_size,) * nd
+    alphas = tuple(pml_alpha) if isinstance(pml_alpha, (tuple, list)) else (pml_alpha,) * nd
+    hdf5.write_attributes(path)
+    hdf5.write_grid(path, kgrid, pml_size=sizes, pml_alpha=alphas)
+    if np.ndim(medium.sound_speed) > 0:
+        hdf5.write_matrix(path, "c0", np.asarray(medium.sound_speed, dtype=np.float64))
+    else:
+        hdf5.write_matrix(path, "c_ref", float(medium.sound_speed))
+    if nd > 1 and np.ndim(medium.density) > 0:
+        hdf5.write_matrix(path, "rho0", np.asarray(medium.density, dtype=np.float64))
+    if source.p0 is not None:
+        hdf5.write_matrix(path, "p0", np.asarray(source.p0, dtype=np.float64))
+    return SimulationOutput({})
+
+
 def kspace_first_order(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KWaveSource,
                        sensor: KWaveSensor, *, pml_inside: bool = True, pml_size=20, pml_alpha=2.0,
                        smooth_p0: bool = True, smooth_c0: bool = False, smooth_rho0: bool = False,
@@ -344,7 +369,7 @@ def kspace_first_order(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source: KW
     rho0 maps in HBM as FP16 offsets; arithmetic stays in ``data_cast``.
     """
     if save_to_disk is not None:
-        raise NotImplementedError("save_to_disk (HDF5 input writer) is not part of this build")
+        return _save_to_disk(save_to_disk, kgrid, medium, source, sensor, pml_size, pml_alpha)
     desc, keep, info = build_desc(kgrid, medium, source, sensor, pml_size=pml_size, pml_alpha=pml_alpha,
                                   smooth_p0=smooth_p0, data_cast=data_cast, medium_storage=medium_storage)
     return _run(desc, keep, info, kgrid, sensor, progress_callback, device)
