@@ -282,6 +282,23 @@ int32_t kw_test_spectral_gradient(kw_ctx* ctx, int32_t precision, int32_t ndim, 
                                   int32_t axis, int32_t sign, const double* f, double* out);
 
 /*
+ * kw_spectral_gradient_noalloc: kw_test_spectral_gradient on device arrays, allocating nothing.
+ * No counterpart in the reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  f and out are device
+ * arrays of prod(N) elements of the precision's type (float or double), column-major; the call is
+ * enqueued on `stream` (a hipStream_t, or NULL for the null stream) and returns without waiting.
+ * The line FFT's twiddles and the derivative multiplier are generated on the device into a
+ * per-device static scratch array of KW_NOALLOC_SCRATCH_BYTES (2 x 4096 complex FP64), whose address
+ * kw_ctx_create resolves; every supported axis length fits.  All calls on one GPU share that array:
+ * calls that overlap on one GPU (different streams or threads) race and must be serialised by the
+ * caller.  Errors as kw_test_spectral_gradient (argument checks run before anything is enqueued).
+ */
+#define KW_NOALLOC_SCRATCH_BYTES 131072
+int32_t kw_spectral_gradient_noalloc(kw_ctx* ctx, int32_t precision, int32_t ndim, const int64_t* N,
+                                     const double* d, int32_t axis, int32_t sign, const void* f, void* out,
+                                     void* stream);
+
+/*
  * Elastic stress-velocity PSTD: replaces the time loops of
  * pstd_elastic_2d (KWave.jl/src/solver/elastic.jl:133-368) and
  * pstd_elastic_3d (elastic.jl:386-588).  Velocities v_a and stresses s_ab

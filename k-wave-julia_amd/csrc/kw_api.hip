@@ -1679,7 +1679,8 @@ int32_t kw_ctx_create(int32_t device, kw_ctx** out) {
   HIPC(hipGetDeviceCount(&n));
   if (device < 0 || device >= n) raise(KW_ERR_INVALID, "device index out of range");
   HIPC(hipSetDevice(device));
-  *out = new kw_ctx{device};
+  void* scratch = noalloc_scratch_address(device);
+  *out = new kw_ctx{device, scratch};
   return KW_OK;
   KW_GUARD_END
 }

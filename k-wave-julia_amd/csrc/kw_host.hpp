@@ -300,10 +300,19 @@ inline void unshifted_derivative(long long n, double h, std::vector<double>& re,
 
 }  // namespace kwh
 
-// Device context behind kw_ctx: owns nothing but the device selection.
+// Device context behind kw_ctx: the device selection and the device address
+// of that device's copy of the allocation-free entry point's scratch array
+// (kw_ops.hip), resolved once at kw_ctx_create.
 struct kw_ctx {
   int device;
+  void* scratch;
 };
+
+namespace kwh {
+// Device address of the allocation-free scratch array on `device` (the
+// current device when called); resolved once per device, then cached.
+void* noalloc_scratch_address(int device);
+}  // namespace kwh
 
 // Every C entry point runs its body inside these: KwError carries the status
 // code, anything else maps to KW_ERR_INVALID; the message goes to kw_last_error().

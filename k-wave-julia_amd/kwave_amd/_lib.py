@@ -39,7 +39,7 @@ EXPORTS = (
     "kw_elastic_create", "kw_elastic_run", "kw_elastic_synchronize", "kw_elastic_steps_done",
     "kw_elastic_get_field", "kw_elastic_get_pressure", "kw_elastic_sensor_shape", "kw_elastic_get_sensor",
     "kw_elastic_profile", "kw_elastic_destroy", "kw_test_axis_table", "kw_test_eos_table",
-    "kw_test_spectral_gradient",
+    "kw_test_spectral_gradient", "kw_spectral_gradient_noalloc",
 )
 AXIS_TABLE = {"k": 0, "pml": 1, "pml_sg": 2, "dplus": 3, "dminus": 4}
 EL_FIELD = {"vx": 0, "vy": 1, "vz": 2, "sxx": 3, "syy": 4, "szz": 5, "sxy": 6, "sxz": 7, "syz": 8}
@@ -204,6 +204,8 @@ def load():
             "kw_test_spectral_gradient": ([P, C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
                                            C.c_int32, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)],
                                           C.c_int32),
+            "kw_spectral_gradient_noalloc": ([P, C.c_int32, C.c_int32, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                              C.c_int32, C.c_int32, P, P, P], C.c_int32),
         }
         for name, (args, res) in sig.items():
             f = getattr(lib, name)
