@@ -213,15 +213,28 @@ def bench_adjoint(a):
     firings = firings[: a.firings]
     med_true = K.KWaveMedium(sound_speed=c_true, density=1000.0)
     med0 = K.KWaveMedium(sound_speed=c_start, density=1000.0)
-    obs = [K.kspace_first_order(kg, med_true, s, r, data_cast=np.float32, **kw)["p"] for s, r in firings]
-    K.gradient_c0(kg, med0, firings[0][0], firings[0][1], obs[0], checkpoint_every=50, data_cast=np.float32, **kw)
+    print(f"[bench] observed data: {len(firings)} stacked forward firings", file=sys.stderr, flush=True)
+    outs = K.kspace_first_order_firings(kg, med_true, [f[0] for f in firings], [f[1] for f in firings],
+                                        data_cast=np.float32, **kw)
+    obs = [o["p"] for o in outs]
+    del outs
+    srcs, sens = [f[0] for f in firings], [f[1] for f in firings]
+    K.gradient_c0(kg, med0, srcs[:2], sens[:2], obs[:2], checkpoint_every=50, data_cast=np.float32, **kw)
+    print(f"[bench] stacked gradient over {len(firings)} firings", file=sys.stderr, flush=True)
     t1 = time.perf_counter()
-    loss_total, g = K.gradient_c0(kg, med0, [f[0] for f in firings], [f[1] for f in firings], obs,
-                                  checkpoint_every=50, data_cast=np.float32, **kw)
+    loss_total, g = K.gradient_c0(kg, med0, srcs, sens, obs, checkpoint_every=50, data_cast=np.float32, **kw)
     t2 = time.perf_counter()
     assert np.all(np.isfinite(g))
     vox = int(np.prod(kg.grid_size()))
     secs = t2 - t1
+    # the same gradient one firing after another (stack=1) on a subset, for the batched/sequential ratio
+    nseq = min(8, len(firings))
+    print(f"[bench] sequential gradient over {nseq} firings", file=sys.stderr, flush=True)
+    t3 = time.perf_counter()
+    _, g_seq = K.gradient_c0(kg, med0, srcs[:nseq], sens[:nseq], obs[:nseq], checkpoint_every=50,
+                             data_cast=np.float32, stack=1, **kw)
+    t4 = time.perf_counter()
+    seq_per_firing = (t4 - t3) / nseq
     # accuracy (untimed): FP64 adjoint of firing 0 against central differences of the FP64
     # forward run on `a.fd_pixels` seeded pixels (h = 1e-3 m/s, dt held), and the FP32
     # adjoint of the same firing against the FP64 one
@@ -257,7 +270,9 @@ def bench_adjoint(a):
                                f"Nt={kg.Nt}, checkpoints every 50 steps; value counts forward time steps "
                                f"(each one forward + one recompute + one adjoint step)",
                    "parallelism": "single GPU"},
-        "seconds_per_firing": secs / len(firings), "loss": loss_total, "fd_check": fd,
+        "seconds_per_firing": secs / len(firings), "sequential_seconds_per_firing": seq_per_firing,
+        "stacked_over_sequential_speedup": seq_per_firing / (secs / len(firings)),
+        "loss": loss_total, "fd_check": fd,
     }
     print(json.dumps(out))
 

@@ -241,6 +241,22 @@ int32_t kw_tune_clear(void);
  */
 int32_t kw_sim_grad_c0(kw_sim* sim, const double* p_obs, int64_t checkpoint_every, double* grad, double* loss);
 
+/*
+ * Stacked firings (the "one simulation per element, batched" form of transmit sequences and of the
+ * adjoint gradient over firings): n_firings 2-D descriptors that share the grid, time step, medium,
+ * PML and record fields and differ in their time-varying sources and binary sensor masks.  They run
+ * as one simulation whose passes cover every firing at once (each firing is an independent 2-D
+ * problem; nothing couples them).  The resulting kw_sim is driven like any other:
+ *   kw_sim_get_sensor / kw_sim_sensor_shape: rows of every firing's sensor points, firing after
+ *     firing (each in its own findall order);
+ *   kw_sim_get_field: Nx x Ny x n_firings;
+ *   kw_sim_grad_c0: p_obs holds those rows (sum of n_sensor) x Nt; grad is Nx x Ny, the sum of the
+ *     firings' gradients; loss is summed.
+ * Source signals of different lengths are merged as the reference samples them (column
+ * min(t, ncols)).  p0, time reversal, post-filters and Cartesian masks are rejected.
+ */
+int32_t kw_sim_create_firings(kw_ctx* ctx, const kw_sim_desc* descs, int32_t n_firings, kw_sim** out);
+
 int32_t kw_sim_destroy(kw_sim* sim);
 
 /*

@@ -72,8 +72,15 @@ template <typename T>
 class Sim final : public SimBase {
  public:
   // batch_mode: fixed workgroup shapes per line length so that a batch can
-  // launch the passes of many simulations together (see Batch below)
-  Sim(int device, const kw_sim_desc& d, bool batch_mode = false) : dev_(device), batch_mode_(batch_mode) { build(d); }
+  // launch the passes of many simulations together (see Batch below).
+  // nfire > 1 (2-D only): `d` holds nfire firings of one grid and medium
+  // stacked along a third, uncoupled axis (maps replicated per firing,
+  // source/sensor indices offset by the firing's plane); every pass runs over
+  // all firings at once and the EOS transform stays 2-D per firing.
+  Sim(int device, const kw_sim_desc& d, bool batch_mode = false, int nfire = 1)
+      : dev_(device), batch_mode_(batch_mode), nfire_(nfire) {
+    build(d);
+  }
   ~Sim() override {
     if (stream_) {
       (void)hipSetDevice(dev_);
@@ -386,13 +393,24 @@ class Sim final : public SimBase {
       HIPC(hipMemcpy(part.data(), dgref.p, ng * sizeof(double), hipMemcpyDeviceToHost));
       double gsum = 0.0;
       for (double v : part) gsum += v;
-      const double gterm = gsum / (double)nvox_;
+      // c_ref = mean(c0) over one firing's grid; with stacked firings each copy
+      // of a voxel takes 1/nfire of the term, so the sum over firings adds it once
+      const double gterm = gsum / (double)plane_ / (double)nfire_;
       const void* pp[6] = {dg.p, has_p0_ ? p0_keep_.p : nullptr, c0sq_.p, Rb[0].p,
                            nd_ > 1 ? Rb[1].p : nullptr, nd_ > 2 ? Rb[2].p : nullptr};
       HIPC(launch_adj_small<T>(4, stream_, pp, &gterm, nvox_, nd_));
     }
     HIPC(hipStreamSynchronize(stream_));
-    HIPC(hipMemcpy(grad, dg.p, nvox_ * sizeof(double), hipMemcpyDeviceToHost));
+    if (nfire_ == 1) {
+      HIPC(hipMemcpy(grad, dg.p, nvox_ * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+      // sum of the firings' gradients, added in firing order
+      std::vector<double> all(nvox_);
+      HIPC(hipMemcpy(all.data(), dg.p, nvox_ * sizeof(double), hipMemcpyDeviceToHost));
+      for (long long i = 0; i < plane_; ++i) grad[i] = all[i];
+      for (int f = 1; f < nfire_; ++f)
+        for (long long i = 0; i < plane_; ++i) grad[i] += all[(size_t)f * plane_ + i];
+    }
     {
       std::vector<double> part(nl);
       HIPC(hipMemcpy(part.data(), dloss.p, nl * sizeof(double), hipMemcpyDeviceToHost));
@@ -454,12 +472,12 @@ class Sim final : public SimBase {
 
   // d/dc_ref [kappa (1 + nabla1)] / N on the octant (heterogeneous branch)
   void build_dkappa(DevBuf& out) {
-    const long long hx = N_[0] / 2 + 1, hy = N_[1] / 2 + 1, hz = N_[2] / 2 + 1;
+    const long long hx = N_[0] / 2 + 1, hy = N_[1] / 2 + 1, hz = nd_ > 2 ? N_[2] / 2 + 1 : 1;
     std::vector<double> kx = wavenumbers(N_[0], d_[0]);
     std::vector<double> ky = nd_ > 1 ? wavenumbers(N_[1], d_[1]) : std::vector<double>{0.0};
     std::vector<double> kz = nd_ > 2 ? wavenumbers(N_[2], d_[2]) : std::vector<double>{0.0};
-    std::vector<T> m(hx * hy * hz);
-    const double inv_n = 1.0 / (double)nvox_;
+    std::vector<T> m((size_t)NHP_ * hy * hz, (T)0);   // rows padded to NHP_ like M1
+    const double inv_n = 1.0 / (double)plane_;
     for (long long iz = 0; iz < hz; ++iz)
       for (long long iy = 0; iy < hy; ++iy)
         for (long long ix = 0; ix < hx; ++ix) {
@@ -472,7 +490,7 @@ class Sim final : public SimBase {
             n1 = tau_ * std::pow(kk, y_);
             dn1 = -2.0 * anp_ * (y_ - 1.0) * std::pow(c_ref_, y_ - 2.0) * std::pow(kk, y_);
           }
-          m[ix + hx * (iz + hz * iy)] = (T)((dkap * (1.0 + n1) + kap * dn1) * inv_n);
+          m[ix + NHP_ * (iz + hz * iy)] = (T)((dkap * (1.0 + n1) + kap * dn1) * inv_n);
         }
     upload(out, m);
   }
@@ -519,6 +537,11 @@ class Sim final : public SimBase {
     nd_ = d.ndim;
     if (nd_ < 1 || nd_ > 3) raise(KW_ERR_INVALID, "ndim must be 1, 2 or 3");
     for (int a = 0; a < 3; ++a) N_[a] = a < nd_ ? d.N[a] : 1;
+    if (nfire_ < 1) raise(KW_ERR_INVALID, "firing count must be positive");
+    if (nfire_ > 1) {
+      if (nd_ != 2) raise(KW_ERR_UNSUPPORTED, "stacked firings need 2-D grids");
+      N_[2] = nfire_;
+    }
     for (int a = 0; a < nd_; ++a) {
       if (N_[a] < 1) raise(KW_ERR_INVALID, "grid size must be positive");
       if (!size_supported(N_[a]))
@@ -545,6 +568,7 @@ class Sim final : public SimBase {
       fill_wgs_ = 2LL * std::max(cus, 1);
     }
     nvox_ = N_[0] * N_[1] * N_[2];
+    plane_ = nvox_ / nfire_;
     NH_ = N_[0] / 2 + 1;
     NHP_ = (NH_ + 15) / 16 * 16;  // 128-byte aligned spectrum rows (FP32)
     dt_ = d.dt;
@@ -686,9 +710,10 @@ class Sim final : public SimBase {
   // first_order_steps.jl:214-290) on the octant |kx| x |ky| x |kz| with the
   // inverse-transform 1/(Nx Ny Nz) folded in.
   void build_eos(const kw_sim_desc& d, bool het_c) {
-    const EosTables E = eos_tables(d, nd_, N_, het_c);
+    const long long Nt3[3] = {N_[0], N_[1], nd_ > 2 ? N_[2] : 1};   // one firing's grid
+    const EosTables E = eos_tables(d, nd_, Nt3, het_c);
     const double c_ref = d.c_ref;
-    const double inv_n = 1.0 / (double)nvox_;
+    const double inv_n = 1.0 / (double)plane_;
     stokes_ = E.stokes;
     c_ref_ = c_ref;
     absorbing_ = E.absorbing;
@@ -696,9 +721,15 @@ class Sim final : public SimBase {
     y_ = d.alpha_power;
     anp_ = E.absorbing ? -E.tau / (2.0 * std::pow(c_ref, y_ - 1.0)) : 0.0;
     for (int q = 0; q < 3; ++q) d_[q] = d.d[q];
-    std::vector<T> m1(E.m1.size()), m2(E.m2.size());
-    for (size_t i = 0; i < m1.size(); ++i) m1[i] = (T)(E.m1[i] * inv_n);
-    for (size_t i = 0; i < m2.size(); ++i) m2[i] = (T)(E.m2[i] * inv_n);
+    // device rows are padded to the spectrum's row pitch (NHP_, a multiple of 16
+    // entries) so a workgroup's run of 16 kx multipliers is one aligned segment
+    const size_t rows = (size_t)(E.hy * E.hz);
+    std::vector<T> m1((size_t)NHP_ * rows, (T)0), m2(E.stokes ? m1.size() : 0, (T)0);
+    for (size_t r = 0; r < rows; ++r)
+      for (long long ix = 0; ix < E.hx; ++ix) {
+        m1[r * NHP_ + ix] = (T)(E.m1[r * E.hx + ix] * inv_n);
+        if (E.stokes) m2[r * NHP_ + ix] = (T)(E.m2[r * E.hx + ix] * inv_n);
+      }
     upload(M1_, m1);
     if (stokes_) upload(M2_, m2);
     // spectra
@@ -956,8 +987,8 @@ class Sim final : public SimBase {
       Y.x_blocks = (int)((NH_ + spec_[0].lpw - 1) / spec_[0].lpw);
       Y.axis = 1;
       Y.Ny = (int)N_[1];
-      Y.Nz = (int)N_[2];
-      Y.MH1 = (int)NH_;
+      Y.Nz = nd_ > 2 ? (int)N_[2] : 1;   // stacked firings: no octant index along the firing axis
+      Y.MH1 = (int)NHP_;   // padded table row pitch
       Y.MH2 = hy;
     }
     if (nd_ > 2) {
@@ -976,7 +1007,7 @@ class Sim final : public SimBase {
       Z.axis = 2;
       Z.Ny = (int)N_[1];
       Z.Nz = (int)N_[2];
-      Z.MH1 = (int)NH_;
+      Z.MH1 = (int)NHP_;
       Z.MH2 = hz;
     }
     std::memset(&ca_, 0, sizeof(ca_));
@@ -1214,6 +1245,8 @@ class Sim final : public SimBase {
 
   int dev_;
   bool batch_mode_ = false;
+  int nfire_ = 1;          // stacked firings (2-D), 1 otherwise
+  long long plane_ = 0;    // voxels of one firing's grid
   hipStream_t stream_ = nullptr;
   int nd_ = 0;
   long long N_[3] = {1, 1, 1};
@@ -1700,6 +1733,170 @@ int32_t kw_grid_supported(int32_t ndim, const int64_t* N, int32_t precision) {
     n[a] = N[a];
   }
   return lane_offsets_fit(ndim, n, precision) ? 1 : 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Host storage of a stacked-firing descriptor: firings of one 2-D grid and
+// medium laid one after another along a third axis.  Maps are replicated per
+// firing; source and sensor indices are offset by the firing's plane; source
+// signals are merged into one matrix whose column count is the longest
+// firing's, shorter signals repeating their last column (the reference samples
+// column min(t, ncols), first_order_steps.jl:561-715).
+struct FiringStack {
+  kw_sim_desc d{};
+  std::vector<double> c0, rho0, bona, psig, usig[3];
+  std::vector<int64_t> pidx, uidx, sidx;
+};
+
+bool same_map(const double* a, const double* b, long long n) {
+  if (a == b) return true;
+  if (!a || !b) return false;
+  return std::memcmp(a, b, (size_t)n * sizeof(double)) == 0;
+}
+
+// Concatenate one index list of every firing (offset by the firing's plane)
+// and, when `sig` is given, its signal matrix (rows x ncols, column-major).
+template <typename Count, typename Index, typename Sig, typename Cols>
+void merge_lists(const kw_sim_desc* ds, int n, long long plane, Count count, Index index, Sig sig, Cols ncols,
+                 bool with_signal, std::vector<int64_t>& idx_out, std::vector<double>& sig_out, long long& ncols_out) {
+  long long total = 0, nc = 0;
+  for (int f = 0; f < n; ++f) {
+    if (count(ds[f]) > 0) nc = std::max(nc, std::max<long long>(ncols(ds[f]), 1));
+    total += count(ds[f]);
+  }
+  idx_out.clear();
+  idx_out.reserve((size_t)total);
+  for (int f = 0; f < n; ++f)
+    for (long long j = 0; j < count(ds[f]); ++j) idx_out.push_back(index(ds[f])[j] + (int64_t)f * plane);
+  ncols_out = nc;
+  if (!with_signal) return;
+  sig_out.assign((size_t)total * (size_t)nc, 0.0);
+  long long row = 0;
+  for (int f = 0; f < n; ++f) {
+    const long long m = count(ds[f]), c_f = std::max<long long>(ncols(ds[f]), 1);
+    const double* sg = sig(ds[f]);
+    if (m > 0 && !sg) raise(KW_ERR_INVALID, "source signal missing");
+    for (long long c = 0; c < nc; ++c) {
+      const long long cs = std::min(c, c_f - 1);
+      for (long long j = 0; j < m; ++j) sig_out[(size_t)c * total + row + j] = sg[cs * m + j];
+    }
+    row += m;
+  }
+}
+
+void stack_firings(const kw_sim_desc* ds, int n, FiringStack& F) {
+  const kw_sim_desc& a = ds[0];
+  if (a.ndim != 2) raise(KW_ERR_UNSUPPORTED, "stacked firings need 2-D grids");
+  const long long plane = a.N[0] * a.N[1];
+  if (a.N[0] < 1 || a.N[1] < 1) raise(KW_ERR_INVALID, "grid size must be positive");
+  for (int f = 0; f < n; ++f) {
+    const kw_sim_desc& b = ds[f];
+    const bool same = b.ndim == a.ndim && b.N[0] == a.N[0] && b.N[1] == a.N[1] && b.d[0] == a.d[0] &&
+                      b.d[1] == a.d[1] && b.dt == a.dt && b.Nt == a.Nt && b.precision == a.precision &&
+                      b.map_storage == a.map_storage && b.pml_size[0] == a.pml_size[0] &&
+                      b.pml_size[1] == a.pml_size[1] && b.pml_alpha[0] == a.pml_alpha[0] &&
+                      b.pml_alpha[1] == a.pml_alpha[1] && b.c_ref == a.c_ref && b.c0_offset == a.c0_offset &&
+                      b.rho0_offset == a.rho0_offset && b.c0_scalar == a.c0_scalar &&
+                      b.rho0_scalar == a.rho0_scalar && b.nonlinear == a.nonlinear &&
+                      b.bona_scalar == a.bona_scalar && b.absorbing == a.absorbing &&
+                      b.alpha_mode == a.alpha_mode && b.alpha_coeff == a.alpha_coeff &&
+                      b.alpha_power == a.alpha_power && b.record == a.record &&
+                      same_map(b.c0, a.c0, plane) && same_map(b.rho0, a.rho0, plane) &&
+                      same_map(b.bona, a.bona, plane);
+    if (!same) raise(KW_ERR_INVALID, "stacked firings must share the grid, time step, medium, PML and record fields");
+    if (b.p0 || b.tr_data || b.directivity || b.freq_response || b.axisymmetric)
+      raise(KW_ERR_UNSUPPORTED, "stacked firings take time-varying sources only (no p0, time reversal, "
+                                "post-filters or axisymmetric grids)");
+    if (b.n_sensor > 0 && !b.sensor_index) raise(KW_ERR_UNSUPPORTED, "stacked firings need binary sensor masks");
+    if (b.n_psrc > 0 && b.p_mode != ds[0].p_mode) raise(KW_ERR_INVALID, "stacked firings need one source mode");
+    if (b.n_usrc > 0 && b.u_mode != ds[0].u_mode) raise(KW_ERR_INVALID, "stacked firings need one source mode");
+    for (int f2 = 0; f2 < f; ++f2)
+      for (int q = 0; q < 3; ++q)
+        if (b.n_usrc > 0 && ds[f2].n_usrc > 0 && (b.usrc_signal[q] == nullptr) != (ds[f2].usrc_signal[q] == nullptr))
+          raise(KW_ERR_INVALID, "stacked firings need the same velocity components");
+  }
+  for (int f = 0; f < n; ++f)
+    for (long long j = 0; j < ds[f].n_psrc + ds[f].n_usrc + ds[f].n_sensor; ++j) {
+      const int64_t* ix = j < ds[f].n_psrc ? ds[f].psrc_index + j
+                          : j < ds[f].n_psrc + ds[f].n_usrc ? ds[f].usrc_index + (j - ds[f].n_psrc)
+                                                            : ds[f].sensor_index + (j - ds[f].n_psrc - ds[f].n_usrc);
+      if (!ix || *ix < 0 || *ix >= plane) raise(KW_ERR_INVALID, "source or sensor index outside the grid");
+    }
+  F.d = a;
+  auto rep = [&](const double* m, std::vector<double>& out) -> const double* {
+    if (!m) return nullptr;
+    out.resize((size_t)plane * n);
+    for (int f = 0; f < n; ++f) std::memcpy(out.data() + (size_t)f * plane, m, (size_t)plane * sizeof(double));
+    return out.data();
+  };
+  F.d.c0 = rep(a.c0, F.c0);
+  F.d.rho0 = rep(a.rho0, F.rho0);
+  F.d.bona = rep(a.bona, F.bona);
+  long long nc = 0;
+  merge_lists(
+      ds, n, plane, [](const kw_sim_desc& x) { return (long long)x.n_psrc; },
+      [](const kw_sim_desc& x) { return x.psrc_index; }, [](const kw_sim_desc& x) { return x.psrc_signal; },
+      [](const kw_sim_desc& x) { return (long long)x.psrc_ncols; }, true, F.pidx, F.psig, nc);
+  F.d.n_psrc = (int64_t)F.pidx.size();
+  F.d.psrc_index = F.pidx.empty() ? nullptr : F.pidx.data();
+  F.d.psrc_signal = F.pidx.empty() ? nullptr : F.psig.data();
+  F.d.psrc_ncols = nc;
+  for (int f = 0; f < n; ++f)
+    if (ds[f].n_psrc > 0) F.d.p_mode = ds[f].p_mode;
+  // velocity sources: the components the firings carry (checked equal above)
+  F.d.n_usrc = 0;
+  F.d.usrc_index = nullptr;
+  int ref = -1;
+  for (int f = n - 1; f >= 0; --f)
+    if (ds[f].n_usrc > 0) ref = f;
+  for (int q = 0; q < 3; ++q) {
+    F.d.usrc_signal[q] = nullptr;
+    F.d.usrc_ncols[q] = 0;
+    if (ref < 0 || !ds[ref].usrc_signal[q]) continue;
+    merge_lists(
+        ds, n, plane, [](const kw_sim_desc& x) { return (long long)x.n_usrc; },
+        [](const kw_sim_desc& x) { return x.usrc_index; }, [q](const kw_sim_desc& x) { return x.usrc_signal[q]; },
+        [q](const kw_sim_desc& x) { return (long long)x.usrc_ncols[q]; }, true, F.uidx, F.usig[q], nc);
+    F.d.usrc_signal[q] = F.usig[q].data();
+    F.d.usrc_ncols[q] = nc;
+  }
+  if (ref >= 0) {
+    F.d.u_mode = ds[ref].u_mode;
+    F.d.n_usrc = (int64_t)F.uidx.size();
+    F.d.usrc_index = F.uidx.data();
+  }
+  std::vector<double> unused;
+  merge_lists(
+      ds, n, plane, [](const kw_sim_desc& x) { return (long long)x.n_sensor; },
+      [](const kw_sim_desc& x) { return x.sensor_index; }, [](const kw_sim_desc&) { return (const double*)nullptr; },
+      [](const kw_sim_desc&) { return 1LL; }, false, F.sidx, unused, nc);
+  F.d.n_sensor = (int64_t)F.sidx.size();
+  F.d.sensor_index = F.sidx.empty() ? nullptr : F.sidx.data();
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t kw_sim_create_firings(kw_ctx* ctx, const kw_sim_desc* descs, int32_t n_firings, kw_sim** out) {
+  KW_GUARD_BEGIN
+  if (!ctx || !descs || !out) raise(KW_ERR_INVALID, "null argument");
+  if (n_firings < 1) raise(KW_ERR_INVALID, "no firings");
+  FiringStack F;
+  stack_firings(descs, n_firings, F);
+  std::unique_ptr<kw_sim> s(new kw_sim);
+  if (F.d.precision == 4)
+    s->impl.reset(new Sim<float>(ctx->device, F.d, false, n_firings));
+  else if (F.d.precision == 8)
+    s->impl.reset(new Sim<double>(ctx->device, F.d, false, n_firings));
+  else
+    raise(KW_ERR_INVALID, "precision must be 4 or 8");
+  *out = s.release();
+  return KW_OK;
+  KW_GUARD_END
 }
 
 int32_t kw_sim_create(kw_ctx* ctx, const kw_sim_desc* desc, kw_sim** out) {

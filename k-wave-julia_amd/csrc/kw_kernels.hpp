@@ -637,7 +637,7 @@ struct SpecArgs {
   // octant table addressing: index = kx + MH1*(|line index| + MH2*|outer index|)
   int axis;           // 1 = line along y, 2 = line along z
   int Ny, Nz;
-  int MH1, MH2;       // Nx/2+1 and the octant extent of the line axis
+  int MH1, MH2;       // table row pitch (>= Nx/2+1) and the octant extent of the line axis
 };
 
 // MODE: 0 forward (both spectra when S2 is set), 1 inverse, 2 fwd*M1*inv,
@@ -695,7 +695,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
     line_fft<T, N, false>(v, lds, tw, t);
     int io = o;
     if (a.axis == 1)
-      io = (a.Nz > 1 && io > a.Nz - io) ? a.Nz - io : io;
+      io = a.Nz > 1 ? (io > a.Nz - io ? a.Nz - io : io) : 0;   // 2-D: outer index = stacked firing
     else
       io = (io > a.Ny - io) ? a.Ny - io : io;
     Cx<T> w[E];
@@ -736,7 +736,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
     // octant index |i| of the outer coordinate (z for y lines, y for z lines)
     int io = o;
     if (a.axis == 1)
-      io = (a.Nz > 1 && io > a.Nz - io) ? a.Nz - io : io;
+      io = a.Nz > 1 ? (io > a.Nz - io ? a.Nz - io : io) : 0;   // 2-D: outer index = stacked firing
     else
       io = (io > a.Ny - io) ? a.Ny - io : io;
 #pragma unroll

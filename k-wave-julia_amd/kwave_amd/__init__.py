@@ -15,7 +15,8 @@ from .builders import make_arc, make_ball, make_bowl, make_circle, make_disc, to
 from .karray import (ArcElement, BowlElement, DiscElement, KWaveArray, RectElement, add_arc_element,
                      add_bowl_element, add_disc_element, add_rect_element, combine_sensor_data,
                      get_array_binary_mask, get_distributed_source_signal, get_element_binary_mask)
-from .solver import Simulation, build_desc, kspace_first_order, kspace_first_order_as, quantize_map
+from .solver import (Simulation, build_desc, kspace_first_order, kspace_first_order_as, kspace_first_order_firings,
+                     quantize_map)
 from .batch import Batch, kspace_first_order_batch
 from .adjoint import gradient_c0, misfit_rrule
 from .elastic import ElasticSimulation, pstd_elastic_2d, pstd_elastic_3d
@@ -32,7 +33,7 @@ __all__ = [
     "KWaveMedium", "KWaveSource", "KWaveSensor", "SimulationOutput", "SourceMode", "Dirichlet",
     "Additive", "AdditiveNoCorrection", "VALID_RECORD_FIELDS", "validate_record_fields",
     "is_homogeneous", "is_lossless", "is_nonlinear", "is_time_reversal", "has_p0",
-    "has_pressure_source", "has_velocity_source", "kspace_first_order", "kspace_first_order_as", "Simulation", "build_desc", "quantize_map",
+    "has_pressure_source", "has_velocity_source", "kspace_first_order", "kspace_first_order_as", "kspace_first_order_firings", "Simulation", "build_desc", "quantize_map",
     "Batch", "kspace_first_order_batch", "gradient_c0", "misfit_rrule", "tuning", "tone_burst", "make_disc", "make_circle", "make_arc", "make_ball", "make_bowl",
     "KWaveArray", "ArcElement", "BowlElement", "DiscElement", "RectElement", "add_arc_element", "add_bowl_element",
     "add_disc_element", "add_rect_element", "get_element_binary_mask", "get_array_binary_mask",

@@ -33,7 +33,7 @@ EXPORTS = (
     "kw_abi_version", "kw_last_error", "kw_device_count", "kw_ctx_create", "kw_ctx_destroy",
     "kw_fft_length_supported", "kw_grid_supported", "kw_sim_create", "kw_sim_run", "kw_sim_synchronize",
     "kw_sim_steps_done", "kw_sim_get_field", "kw_sim_sensor_shape", "kw_sim_get_sensor",
-    "kw_sim_profile", "kw_sim_grad_c0", "kw_sim_destroy", "kw_autotune", "kw_tune_export", "kw_tune_import",
+    "kw_sim_profile", "kw_sim_grad_c0", "kw_sim_create_firings", "kw_sim_destroy", "kw_autotune", "kw_tune_export", "kw_tune_import",
     "kw_tune_clear", "kw_batch_create", "kw_batch_run", "kw_batch_synchronize",
     "kw_batch_sensor_shape", "kw_batch_get_sensor", "kw_batch_get_field", "kw_batch_destroy",
     "kw_elastic_create", "kw_elastic_run", "kw_elastic_synchronize", "kw_elastic_steps_done",
@@ -174,6 +174,7 @@ def load():
                                 C.POINTER(C.c_double), C.POINTER(C.c_int64)], C.c_int32),
             "kw_sim_grad_c0": ([P, C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_double),
                                 C.POINTER(C.c_double)], C.c_int32),
+            "kw_sim_create_firings": ([P, C.POINTER(SimDesc), C.c_int32, C.POINTER(P)], C.c_int32),
             "kw_sim_destroy": ([P], C.c_int32),
             "kw_autotune": ([P, C.POINTER(SimDesc), C.c_int32, C.c_int32], C.c_int32),
             "kw_tune_export": ([C.c_char_p, C.c_int64, C.POINTER(C.c_int64)], C.c_int32),

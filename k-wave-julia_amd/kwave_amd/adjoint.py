@@ -13,7 +13,7 @@ a pullback, the shape a ChainRulesCore ``rrule`` has on the Julia side.
 """
 from __future__ import annotations
 
-from typing import Callable, Sequence, Tuple, Union
+from typing import Callable, Optional, Sequence, Tuple, Union
 
 import numpy as np
 
@@ -31,26 +31,47 @@ def _firings(source, sensor, p_obs):
 
 
 def gradient_c0(kgrid, medium: KWaveMedium, source, sensor, p_obs, *, checkpoint_every: int = 50,
-                data_cast=np.float32, device: int = 0, **kwargs) -> Tuple[float, np.ndarray]:
+                data_cast=np.float32, device: int = 0, stack: Optional[int] = None,
+                **kwargs) -> Tuple[float, np.ndarray]:
     """Misfit and its gradient w.r.t. ``medium.sound_speed`` (a map).
 
     ``source``/``sensor``/``p_obs`` may be lists (one entry per firing; the
     losses and gradients are summed).  Each ``p_obs`` is n_sensor x Nt.
     Keyword arguments are those of ``kspace_first_order`` (pml_size,
     pml_alpha, smooth_p0, ...).  kgrid.dt and kgrid.Nt are used as set.
+
+    On 2-D grids the firings run stacked (kw_sim_create_firings): every pass
+    of the forward, recompute and adjoint sweeps covers ``stack`` firings at
+    once (default: all of them); ``stack=1`` runs them one after another.
     """
     c0 = np.asarray(medium.sound_speed, dtype=np.float64)
     if c0.ndim == 0:
         raise ValueError("gradient_c0 needs medium.sound_speed as a map")
-    total, grad = 0.0, np.zeros(c0.shape)
-    for src, sen, obs in _firings(source, sensor, p_obs):
+    firings = _firings(source, sensor, p_obs)
+    prepared = []
+    for src, sen, obs in firings:
         if "p" not in sen.record:
             raise ValueError("the misfit needs 'p' in sensor.record")
         desc, keep, info = build_desc(kgrid, medium, src, sen, data_cast=data_cast, **kwargs)
         obs = np.asarray(obs, dtype=np.float64)
         if obs.shape != (info["n_sensor"], info["Nt"]):
             raise ValueError(f"p_obs must be {(info['n_sensor'], info['Nt'])}, got {obs.shape}")
-        with Simulation(desc, keep, device=device) as s:
+        prepared.append((desc, keep, info, obs, src))
+    stacked = kgrid.ndim == 2 and len(prepared) > 1 and (stack is None or stack > 1) and all(
+        x[4].p0 is None for x in prepared)
+    total, grad = 0.0, np.zeros(c0.shape)
+    if not stacked:
+        for desc, keep, info, obs, _ in prepared:
+            with Simulation(desc, keep, device=device) as s:
+                loss, g = s.grad_c0(obs, checkpoint_every)
+            total += loss
+            grad += g
+        return total, grad
+    size = len(prepared) if stack is None else int(stack)
+    for i in range(0, len(prepared), size):
+        group = prepared[i:i + size]
+        obs = np.concatenate([x[3] for x in group], axis=0)
+        with Simulation.firings([x[0] for x in group], [x[1] for x in group], device=device) as s:
             loss, g = s.grad_c0(obs, checkpoint_every)
         total += loss
         grad += g

@@ -65,16 +65,33 @@ def _pml_alpha_cast(x):
 
 
 class Simulation:
-    """One GPU simulation behind the C ABI (kw_sim_create .. kw_sim_destroy)."""
+    """One GPU simulation behind the C ABI (kw_sim_create .. kw_sim_destroy).
 
-    def __init__(self, desc: "_lib.SimDesc", keep: List[object], device: int = 0):
+    ``Simulation.firings(descs, keeps)`` builds the stacked form
+    (kw_sim_create_firings): 2-D firings of one grid and medium that run as one
+    simulation; fields then have a trailing firing axis and sensor rows are
+    every firing's rows in turn."""
+
+    def __init__(self, desc: "_lib.SimDesc", keep: List[object], device: int = 0, _firings=None):
         self._lib = _lib.load()
         ctx = _lib.context(device)
         h = C.c_void_p()
-        _lib.check(self._lib.kw_sim_create(ctx, C.byref(desc), C.byref(h)))
+        if _firings is None:
+            _lib.check(self._lib.kw_sim_create(ctx, C.byref(desc), C.byref(h)))
+            self.n_firings = 1
+        else:
+            arr = (_lib.SimDesc * len(_firings))(*_firings)
+            _lib.check(self._lib.kw_sim_create_firings(ctx, arr, len(_firings), C.byref(h)))
+            self.n_firings = len(_firings)
         self._h = h
         self.desc = desc
         del keep  # host inputs are borrowed only during kw_sim_create
+
+    @classmethod
+    def firings(cls, descs: Sequence["_lib.SimDesc"], keeps: Sequence[object], device: int = 0) -> "Simulation":
+        if not descs:
+            raise ValueError("no firings")
+        return cls(descs[0], list(keeps), device, _firings=list(descs))
 
     def run(self, n_steps: int) -> None:
         _lib.check(self._lib.kw_sim_run(self._h, int(n_steps)))
@@ -89,7 +106,7 @@ class Simulation:
 
     def field(self, name: str) -> np.ndarray:
         nd = self.desc.ndim
-        shape = tuple(self.desc.N[a] for a in range(nd))
+        shape = tuple(self.desc.N[a] for a in range(nd)) + ((self.n_firings,) if self.n_firings > 1 else ())
         out = np.empty(int(np.prod(shape)), dtype=np.float64)
         _lib.check(self._lib.kw_sim_get_field(self._h, _lib.FIELD[name], out.ctypes.data_as(_DP)))
         return out.reshape(shape, order="F")
@@ -393,6 +410,54 @@ def kspace_first_order_as(kgrid: AbstractKWaveGrid, medium: KWaveMedium, source:
                                   smooth_p0=smooth_p0, data_cast=data_cast, medium_storage=medium_storage,
                                   axisymmetric=True)
     return _run(desc, keep, info, kgrid, sensor, progress_callback, device)
+
+
+def kspace_first_order_firings(kgrid: AbstractKWaveGrid, medium: KWaveMedium, sources: Sequence[KWaveSource],
+                               sensors, *, pml_size=20, pml_alpha=2.0, data_cast=np.float64, medium_storage=None,
+                               device: int = 0, **ignored) -> List[SimulationOutput]:
+    """Transmit sequence: one 2-D simulation per source (firing) on one grid and medium,
+    run as one stacked simulation on the GPU (kw_sim_create_firings).
+
+    Returns one SimulationOutput per firing, equal to
+    ``kspace_first_order(kgrid, medium, sources[i], sensors[i], ...)``.  ``sensors`` is one
+    sensor for all firings or one per firing (binary masks).  Time-varying sources only:
+    p0, time reversal and sensor post-filters are not part of this form.
+    """
+    if kgrid.ndim != 2:
+        raise TypeError("kspace_first_order_firings takes a 2-D grid")
+    sources = list(sources)
+    sens = list(sensors) if isinstance(sensors, (list, tuple)) else [sensors] * len(sources)
+    if len(sens) != len(sources) or not sources:
+        raise ValueError("one sensor per firing (or one sensor for all) and at least one firing")
+    descs, keeps, infos = [], [], []
+    for src, sen in zip(sources, sens):
+        if src.p0 is not None:
+            raise ValueError("kspace_first_order_firings takes time-varying sources (no p0)")
+        d, k, info = build_desc(kgrid, medium, src, sen, pml_size=pml_size, pml_alpha=pml_alpha, smooth_p0=False,
+                                data_cast=data_cast, medium_storage=medium_storage)
+        if info["cartesian"] or info["tr"]:
+            raise ValueError("kspace_first_order_firings needs binary sensor masks and no time reversal")
+        descs.append(d)
+        keeps.append(k)
+        infos.append(info)
+    Nt = infos[0]["Nt"]
+    with Simulation.firings(descs, keeps, device) as sim:
+        sim.run(Nt)
+        sim.synchronize()
+        outs = [dict() for _ in sources]
+        record = list(sens[0].record) if sens[0].mask is not None else []
+        for f in record:
+            if f in ("uz",):
+                for i, info in enumerate(infos):
+                    outs[i][f] = np.zeros((info["n_sensor"], Nt))
+                continue
+            whole = sim.sensor(f)
+            row = 0
+            for i, info in enumerate(infos):
+                n = info["n_sensor"]
+                outs[i][f] = whole[row:row + n]
+                row += n
+    return [SimulationOutput(o) for o in outs]
 
 
 def _run(desc, keep, info, kgrid, sensor, progress_callback, device) -> SimulationOutput:
