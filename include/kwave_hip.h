@@ -62,8 +62,11 @@ enum kw_record {
   KW_REC_I_MAX = 1u << 12
 };
 
-/* Fields readable with kw_sim_get_field. */
-enum kw_field { KW_FIELD_P = 0, KW_FIELD_UX, KW_FIELD_UY, KW_FIELD_UZ, KW_FIELD_RHOX, KW_FIELD_RHOY, KW_FIELD_RHOZ };
+/* Fields read by kw_sim_get_field and written by kw_sim_set_field.  KW_FIELD_RHO_PREV is the
+ * previous total density of the homogeneous :stokes branch (rho_total_prev,
+ * first_order_steps.jl:255-259, 356-358), present only when that branch runs. */
+enum kw_field { KW_FIELD_P = 0, KW_FIELD_UX, KW_FIELD_UY, KW_FIELD_UZ, KW_FIELD_RHOX, KW_FIELD_RHOY, KW_FIELD_RHOZ,
+                KW_FIELD_RHO_PREV };
 
 /* Storage of the static medium maps (c0, rho0, B/A). */
 enum kw_map_storage { KW_MAPS_NATIVE = 0, KW_MAPS_FP16 = 2 };
@@ -148,6 +151,10 @@ typedef struct kw_sim_desc {
 /* Library identity and errors. */
 int32_t kw_abi_version(void);
 const char* kw_last_error(void);
+/* sizeof(kw_sim_desc) and sizeof(kw_elastic_desc) as this library was compiled: a binding
+ * checks them against its own structure layout when it loads the library (a library and a
+ * binding built from different versions of this header would otherwise misread pointers). */
+int32_t kw_desc_sizes(int64_t* sim_desc_bytes, int64_t* elastic_desc_bytes);
 
 /* Device context: owns nothing but the device selection. */
 int32_t kw_device_count(int32_t* count);
@@ -184,6 +191,11 @@ int32_t kw_sim_steps_done(const kw_sim* sim, int64_t* steps);
 
 /* Copy a full field (column-major, Float64) into caller memory. */
 int32_t kw_sim_get_field(kw_sim* sim, int32_t which, double* out);
+/* Overwrite a full field of the current state (Float64 in, cast to the kernel precision):
+ * with kw_sim_run(sim, 1) this advances one time_step_{1,2,3}d! (first_order_steps.jl:296-535)
+ * from a given state -- the operator-level form of the time step.  The step counter is not
+ * changed (sources and recording continue from steps_done + 1). */
+int32_t kw_sim_set_field(kw_sim* sim, int32_t which, const double* in);
 
 /*
  * Finalised sensor data (finalize_sensor_data!, first_order_steps.jl:818-825)

@@ -44,6 +44,11 @@ function lib()
         _lib[] = Libdl.dlopen(libpath())
         v = ccall(Libdl.dlsym(_lib[], :kw_abi_version), Int32, ())
         v == 1 || error("libkwave_hip ABI version mismatch ($v)")
+        # a library built from another version of kwave_hip.h would misread the descriptor
+        ns, ne = Ref{Int64}(0), Ref{Int64}(0)
+        ccall(Libdl.dlsym(_lib[], :kw_desc_sizes), Int32, (Ref{Int64}, Ref{Int64}), ns, ne)
+        ns[] == sizeof(KwSimDesc) || error("$(libpath()) was built from a different kwave_hip.h " *
+                                           "(kw_sim_desc is $(ns[]) bytes there, $(sizeof(KwSimDesc)) here): rebuild it")
     end
     return _lib[]
 end
@@ -71,7 +76,8 @@ const REC = Dict{Symbol, UInt32}(:p => 1 << 0, :p_max => 1 << 1, :p_min => 1 << 
                                  :u_max => 1 << 8, :u_rms => 1 << 9, :u_final => 1 << 10,
                                  :I_avg => 1 << 11, :I_max => 1 << 12)
 const VALID_RECORD_FIELDS = Set(keys(REC))
-const FIELD = Dict{Symbol, Int32}(:p => 0, :ux => 1, :uy => 2, :uz => 3, :rhox => 4, :rhoy => 5, :rhoz => 6)
+const FIELD = Dict{Symbol, Int32}(:p => 0, :ux => 1, :uy => 2, :uz => 3, :rhox => 4, :rhoy => 5, :rhoz => 6,
+                                   :rho_prev => 7)
 # alpha_mode symbols (medium.jl:66); any other symbol acts as :stokes (first_order_steps.jl:252)
 alpha_mode_code(m::Symbol) = m === :no_absorption ? Int32(0) : (m === :no_dispersion ? Int32(1) : Int32(2))
 
@@ -352,6 +358,18 @@ function Simulation(desc::KwSimDesc, keep; device::Integer=0)
     return s
 end
 
+# Stacked firings (kw_sim_create_firings): 2-D descriptors sharing grid and medium, run as one simulation
+function Simulation(descs::Vector{KwSimDesc}, keeps; device::Integer=0)
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve descs keeps begin
+        check(ccall(sym(:kw_sim_create_firings), Int32, (Ptr{Cvoid}, Ptr{KwSimDesc}, Int32, Ref{Ptr{Cvoid}}),
+                    context(device), descs, Int32(length(descs)), h))
+    end
+    s = Simulation(h[], descs[1])
+    finalizer(close!, s)
+    return s
+end
+
 run!(s::Simulation, n::Integer) = check(ccall(sym(:kw_sim_run), Int32, (Ptr{Cvoid}, Int64), s.h, n))
 synchronize(s::Simulation) = check(ccall(sym(:kw_sim_synchronize), Int32, (Ptr{Cvoid},), s.h))
 function close!(s::Simulation)
@@ -368,6 +386,14 @@ function field(s::Simulation, name::Symbol)
     out = Vector{Float64}(undef, prod(_shape(s)))
     check(ccall(sym(:kw_sim_get_field), Int32, (Ptr{Cvoid}, Int32, Ptr{Float64}), s.h, FIELD[name], out))
     return reshape(out, _shape(s))
+end
+
+# overwrite one field of the state (kw_sim_set_field); with run!(s, 1): one time step from that state
+function set_field!(s::Simulation, name::Symbol, a::AbstractArray)
+    size(a) == _shape(s) || error("field must have size $(_shape(s))")
+    v = Vector{Float64}(vec(a))
+    check(ccall(sym(:kw_sim_set_field), Int32, (Ptr{Cvoid}, Int32, Ptr{Float64}), s.h, FIELD[name], v))
+    return nothing
 end
 
 function sensor_data(s::Simulation, name::Symbol)
@@ -457,24 +483,36 @@ kspace_first_order_as(kgrid::AbstractKWaveGrid, medium, source, sensor; kwargs..
 Loss 1/2 sum (p - p_obs)^2 over the recorded p of every firing and its
 gradient with respect to `medium.sound_speed` by the checkpointed discrete
 adjoint (`kw_sim_grad_c0`); `sources`, `sensors`, `p_obs` may be vectors (one
-entry per firing), whose losses and gradients are summed.
+entry per firing), whose losses and gradients are summed.  2-D firings run
+stacked (`kw_sim_create_firings`, `stack` firings per simulation, default all;
+`stack=1` one at a time).
 """
 function gradient_c0(kgrid, medium, source, sensor, p_obs; checkpoint_every::Integer=50,
-                     data_cast::Type=Float32, device::Integer=0, kwargs...)
+                     data_cast::Type=Float32, device::Integer=0, stack::Union{Nothing, Integer}=nothing, kwargs...)
     medium.sound_speed isa AbstractArray || error("gradient_c0 needs medium.sound_speed as a map")
     firings = source isa AbstractVector ?
               collect(zip(source, sensor isa AbstractVector ? sensor : fill(sensor, length(source)), p_obs)) :
               [(source, sensor, p_obs)]
     total = 0.0
     grad = zeros(Float64, size(medium.sound_speed))
-    for (src, sen, obs) in firings
+    prepared = map(firings) do (src, sen, obs)
         :p in sen.record || error("the misfit needs :p in sensor.record")
         desc, keep, info = fill_desc(kgrid, medium, src, sen; data_cast=data_cast, kwargs...)
         size(obs) == (info[:n_sensor], info[:Nt]) || error("p_obs must be $((info[:n_sensor], info[:Nt]))")
-        o = Matrix{Float64}(obs)
+        (desc, keep, Matrix{Float64}(obs), src)
+    end
+    # 2-D firings without p0 run stacked: one simulation whose passes cover `stack` firings at once
+    stacked = ndims(medium.sound_speed) == 2 && length(prepared) > 1 && stack != 1 &&
+              all(x -> x[4].p0 === nothing, prepared)
+    groups = stacked ? Iterators.partition(prepared, stack === nothing ? length(prepared) : stack) :
+             ([x] for x in prepared)
+    for grp in groups
+        grp = collect(grp)
+        o = reduce(vcat, [x[3] for x in grp])
         g = Vector{Float64}(undef, length(grad))
         loss = Ref{Float64}(0.0)
-        sim = Simulation(desc, keep; device=device)
+        sim = length(grp) == 1 && !stacked ? Simulation(grp[1][1], grp[1][2]; device=device) :
+              Simulation([x[1] for x in grp], [x[2] for x in grp]; device=device)
         try
             check(ccall(sym(:kw_sim_grad_c0), Int32, (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Float64}, Ref{Float64}),
                         sim.h, o, checkpoint_every, g, loss))

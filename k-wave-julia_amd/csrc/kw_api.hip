@@ -57,6 +57,7 @@ struct SimBase {
   virtual void sync() = 0;
   virtual long long steps_done() const = 0;
   virtual void get_field(int which, double* out) = 0;
+  virtual void set_field(int which, const double* in) = 0;
   virtual void sensor_shape(uint32_t f, long long* r, long long* c) = 0;
   virtual void get_sensor(uint32_t f, double* out) = 0;
   virtual void profile(long long n, int maxc, int* nc, const char** names, double* ms, long long* launches) = 0;
@@ -101,18 +102,31 @@ class Sim final : public SimBase {
   }
   long long steps_done() const override { return t_; }
 
+  // device buffer of a field selector (kw_field), or null if the selector is invalid here
+  DevBuf* field_buf(int which) {
+    if (which == KW_FIELD_P) return &p_;
+    if (which >= KW_FIELD_UX && which <= KW_FIELD_UZ && which - KW_FIELD_UX < nd_) return &u_[which - KW_FIELD_UX];
+    if (which >= KW_FIELD_RHOX && which <= KW_FIELD_RHOZ && which - KW_FIELD_RHOX < nd_)
+      return &rho_[which - KW_FIELD_RHOX];
+    if (which == KW_FIELD_RHO_PREV && stokes_ && rho_prev_.p) return &rho_prev_;
+    return nullptr;
+  }
+
   void get_field(int which, double* out) override {
     HIPC(hipSetDevice(dev_));
-    const T* src = nullptr;
-    if (which == KW_FIELD_P)
-      src = p_.as<T>();
-    else if (which >= KW_FIELD_UX && which <= KW_FIELD_UZ && which - KW_FIELD_UX < nd_)
-      src = u_[which - KW_FIELD_UX].as<T>();
-    else if (which >= KW_FIELD_RHOX && which <= KW_FIELD_RHOZ && which - KW_FIELD_RHOX < nd_)
-      src = rho_[which - KW_FIELD_RHOX].as<T>();
-    else
-      raise(KW_ERR_INVALID, "invalid field selector");
-    copy_out(src, nvox_, out);
+    DevBuf* b = field_buf(which);
+    if (!b) raise(KW_ERR_INVALID, "invalid field selector");
+    copy_out(b->as<T>(), nvox_, out);
+  }
+
+  void set_field(int which, const double* in) override {
+    HIPC(hipSetDevice(dev_));
+    DevBuf* b = field_buf(which);
+    if (!b) raise(KW_ERR_INVALID, "invalid field selector");
+    HIPC(hipStreamSynchronize(stream_));
+    std::vector<T> h(nvox_);
+    for (long long i = 0; i < nvox_; ++i) h[i] = (T)in[i];
+    HIPC(hipMemcpy(b->p, h.data(), nvox_ * sizeof(T), hipMemcpyHostToDevice));
   }
 
   void sensor_shape(uint32_t f, long long* r, long long* c) override {
@@ -1725,6 +1739,12 @@ int32_t kw_ctx_destroy(kw_ctx* ctx) {
 
 int32_t kw_fft_length_supported(int64_t n) { return size_supported(n) ? 1 : 0; }
 
+int32_t kw_desc_sizes(int64_t* sim_desc_bytes, int64_t* elastic_desc_bytes) {
+  if (sim_desc_bytes) *sim_desc_bytes = (int64_t)sizeof(kw_sim_desc);
+  if (elastic_desc_bytes) *elastic_desc_bytes = (int64_t)sizeof(kw_elastic_desc);
+  return KW_OK;
+}
+
 int32_t kw_grid_supported(int32_t ndim, const int64_t* N, int32_t precision) {
   if (ndim < 1 || ndim > 3 || N == nullptr || (precision != 4 && precision != 8)) return 0;
   long long n[3] = {1, 1, 1};
@@ -1943,6 +1963,14 @@ int32_t kw_sim_get_field(kw_sim* sim, int32_t which, double* out) {
   KW_GUARD_BEGIN
   if (!sim || !out) raise(KW_ERR_INVALID, "null argument");
   sim->impl->get_field(which, out);
+  return KW_OK;
+  KW_GUARD_END
+}
+
+int32_t kw_sim_set_field(kw_sim* sim, int32_t which, const double* in) {
+  KW_GUARD_BEGIN
+  if (!sim || !in) raise(KW_ERR_INVALID, "null argument");
+  sim->impl->set_field(which, in);
   return KW_OK;
   KW_GUARD_END
 }

@@ -25,14 +25,14 @@ REC_BITS = {
     "ux": 1 << 5, "uy": 1 << 6, "uz": 1 << 7, "u_max": 1 << 8, "u_rms": 1 << 9,
     "u_final": 1 << 10, "I_avg": 1 << 11, "I_max": 1 << 12,
 }
-FIELD = {"p": 0, "ux": 1, "uy": 2, "uz": 3, "rhox": 4, "rhoy": 5, "rhoz": 6}
+FIELD = {"p": 0, "ux": 1, "uy": 2, "uz": 3, "rhox": 4, "rhoy": 5, "rhoz": 6, "rho_prev": 7}
 ALPHA_MODE = {"no_absorption": 0, "no_dispersion": 1, "stokes": 2}
 
 # every symbol include/kwave_hip.h declares
 EXPORTS = (
-    "kw_abi_version", "kw_last_error", "kw_device_count", "kw_ctx_create", "kw_ctx_destroy",
+    "kw_abi_version", "kw_last_error", "kw_desc_sizes", "kw_device_count", "kw_ctx_create", "kw_ctx_destroy",
     "kw_fft_length_supported", "kw_grid_supported", "kw_sim_create", "kw_sim_run", "kw_sim_synchronize",
-    "kw_sim_steps_done", "kw_sim_get_field", "kw_sim_sensor_shape", "kw_sim_get_sensor",
+    "kw_sim_steps_done", "kw_sim_get_field", "kw_sim_set_field", "kw_sim_sensor_shape", "kw_sim_get_sensor",
     "kw_sim_profile", "kw_sim_grad_c0", "kw_sim_create_firings", "kw_sim_destroy", "kw_autotune", "kw_tune_export", "kw_tune_import",
     "kw_tune_clear", "kw_batch_create", "kw_batch_run", "kw_batch_synchronize",
     "kw_batch_sensor_shape", "kw_batch_get_sensor", "kw_batch_get_field", "kw_batch_destroy",
@@ -158,6 +158,7 @@ def load():
         sig = {
             "kw_abi_version": ([], C.c_int32),
             "kw_last_error": ([], C.c_char_p),
+            "kw_desc_sizes": ([C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int32),
             "kw_device_count": ([C.POINTER(C.c_int32)], C.c_int32),
             "kw_ctx_create": ([C.c_int32, C.POINTER(P)], C.c_int32),
             "kw_ctx_destroy": ([P], C.c_int32),
@@ -168,6 +169,7 @@ def load():
             "kw_sim_synchronize": ([P], C.c_int32),
             "kw_sim_steps_done": ([P, C.POINTER(C.c_int64)], C.c_int32),
             "kw_sim_get_field": ([P, C.c_int32, C.POINTER(C.c_double)], C.c_int32),
+            "kw_sim_set_field": ([P, C.c_int32, C.POINTER(C.c_double)], C.c_int32),
             "kw_sim_sensor_shape": ([P, C.c_uint32, C.POINTER(C.c_int64), C.POINTER(C.c_int64)], C.c_int32),
             "kw_sim_get_sensor": ([P, C.c_uint32, C.POINTER(C.c_double)], C.c_int32),
             "kw_sim_profile": ([P, C.c_int64, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_char_p),
@@ -214,6 +216,14 @@ def load():
             f.restype = res
         if lib.kw_abi_version() != 1:
             raise KWaveHIPError(KW_ERR_STATE, "libkwave_hip ABI version mismatch")
+        # a library built from another version of include/kwave_hip.h would misread the
+        # descriptors' pointers: refuse it here instead of crashing inside kw_sim_create
+        ns, ne = C.c_int64(), C.c_int64()
+        lib.kw_desc_sizes(C.byref(ns), C.byref(ne))
+        if (ns.value, ne.value) != (C.sizeof(SimDesc), C.sizeof(ElasticDesc)):
+            raise KWaveHIPError(KW_ERR_STATE, f"{LIB_PATH} was built from a different include/kwave_hip.h "
+                                f"(descriptor sizes {ns.value}/{ne.value} bytes, this binding "
+                                f"{C.sizeof(SimDesc)}/{C.sizeof(ElasticDesc)}): rebuild it")
         _lib = lib
         return lib
 

@@ -111,6 +111,17 @@ class Simulation:
         _lib.check(self._lib.kw_sim_get_field(self._h, _lib.FIELD[name], out.ctypes.data_as(_DP)))
         return out.reshape(shape, order="F")
 
+    def set_field(self, name: str, value) -> None:
+        """Overwrite one field of the current state (p, ux.., rhox.., rho_prev); with
+        ``run(1)`` this advances one time step from a given state."""
+        nd = self.desc.ndim
+        shape = tuple(self.desc.N[a] for a in range(nd)) + ((self.n_firings,) if self.n_firings > 1 else ())
+        a = np.asarray(value, dtype=np.float64)
+        if a.shape != shape:
+            raise ValueError(f"field must have shape {shape}, got {a.shape}")
+        buf = np.ascontiguousarray(a.ravel(order="F"))
+        _lib.check(self._lib.kw_sim_set_field(self._h, _lib.FIELD[name], buf.ctypes.data_as(_DP)))
+
     def sensor(self, name: str) -> np.ndarray:
         bit = _lib.REC_BITS[name]
         r, c = C.c_int64(), C.c_int64()

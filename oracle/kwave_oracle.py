@@ -427,13 +427,17 @@ def _norm_tuple(v, nd, cast):
 def kspace_first_order(kgrid, medium, source, sensor, *, pml_size=20, pml_alpha=2.0,
                        smooth_p0=True, data_cast=np.float64, progress_callback=None,
                        n_steps: Optional[int] = None, axisymmetric: bool = False,
+                       initial_state: Optional[Dict[str, Any]] = None,
                        **_ignored) -> Dict[str, np.ndarray]:
     """Restatement of kspace_first_order 1D/2D/3D (first_order.jl:92-281, 287-499, 505-728).
 
     ``kgrid``/``medium``/``source``/``sensor`` are duck-typed objects with the
     reference's field names.  ``n_steps`` (oracle-only) truncates the loop for
     timing samples.  ``axisymmetric`` selects the kspace_first_order_as loop
-    (see that function).  Returns the sensor-data dict (Float64 arrays).
+    (see that function).  ``initial_state`` (oracle-only: dict with ``p``,
+    ``u`` and ``rho`` lists per axis, optional ``rho_prev``) replaces the
+    state before step 1, so that n_steps=1 is one time_step_{1,2,3}d! from
+    that state.  Returns the sensor-data dict (Float64 arrays).
     """
     g = grid_from(kgrid)
     nd = g.ndim
@@ -536,6 +540,14 @@ def kspace_first_order(kgrid, medium, source, sensor, *, pml_size=20, pml_alpha=
             dp = spectral_gradient(p0, g.k_vecs[a], g.shifts[a][0], a)
             r = rho0 if rho0_sg is None else stagger_density(rho0, a)
             u[a][...] = -dt / (2 * r) * dp
+
+    if initial_state is not None:
+        p[...] = np.asarray(initial_state["p"], dtype=np.float64).astype(T)
+        for a in range(nd):
+            u[a][...] = np.asarray(initial_state["u"][a], dtype=np.float64).astype(T)
+            rho[a][...] = np.asarray(initial_state["rho"][a], dtype=np.float64).astype(T)
+        if rho_total_prev is not None and initial_state.get("rho_prev") is not None:
+            rho_total_prev[...] = np.asarray(initial_state["rho_prev"], dtype=np.float64).astype(T)
 
     # sources (first_order_steps.jl:541-715)
     p_mask = getattr(source, "p_mask", None)

@@ -361,3 +361,28 @@ def test_smooth_window_and_restore_max():
     assert np.max(np.abs(s)) == pytest.approx(np.max(np.abs(x)))
     c = np.full((16, 13), 2.5)
     assert np.allclose(O.smooth(c), c * O.blackman_window(16)[8] * O.blackman_window(13)[7])
+
+
+def test_initial_state_hook_continues_a_run():
+    """The oracle's initial_state hook: the state after 9 steps of a p0 run, stepped once through
+    the hook, equals step 10 of the uninterrupted run (no time-indexed sources; heterogeneous c0,
+    so no previous-density state)."""
+    kg = K.KWaveGrid(32, 1e-4, 24, 1e-4)
+    c0 = np.full((32, 24), 1500.0)
+    c0[18:, :] = 1620.0
+    med = K.KWaveMedium(sound_speed=c0, density=1000.0, alpha_coeff=1.0, alpha_power=1.5)
+    p0 = np.zeros((32, 24))
+    p0[12:16, 8:12] = 1.0
+    K.make_time(kg, 1620.0)
+    src = K.KWaveSource(p0=p0)
+    sen = K.KWaveSensor(mask=np.ones((32, 24), dtype=bool), record=["p"])
+    kw = dict(pml_size=6, pml_alpha=2.0, smooth_p0=False)
+    full = O.kspace_first_order(kg, med, src, sen, n_steps=10, **kw)["_state"]
+    part = O.kspace_first_order(kg, med, src, sen, n_steps=9, **kw)["_state"]
+    st = {"p": part["p"], "u": part["u"], "rho": part["rho"]}
+    one = O.kspace_first_order(kg, med, K.KWaveSource(), sen, n_steps=1, initial_state=st, **kw)["_state"]
+    for k in ("p",):
+        np.testing.assert_allclose(one[k], full[k], rtol=0, atol=1e-12 * np.abs(full[k]).max())
+    for a in range(2):
+        np.testing.assert_allclose(one["u"][a], full["u"][a], rtol=0, atol=1e-12 * np.abs(full["u"][a]).max())
+        np.testing.assert_allclose(one["rho"][a], full["rho"][a], rtol=0, atol=1e-12 * np.abs(full["rho"][a]).max())
