@@ -397,7 +397,22 @@ __device__ __forceinline__ Twiddles<T, N, V> load_twiddles(const Cx<T>* __restri
   return tw;
 }
 
-template <typename T, int N, int V, int TPL, bool INV, int Ns, int OFF, int R, int... Rest>
+// LDS exchange synchronisation.  WAVE: every line of the workgroup lies within
+// one wavefront (contiguous lines of TPL threads, TPL dividing 64), so the
+// exchange needs only an ordering fence for the compiler and the wave's own
+// in-order LDS queue, not a workgroup barrier.
+template <bool WAVE>
+__device__ __forceinline__ void lds_sync() {
+  if constexpr (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
+template <typename T, int N, int V, int TPL, bool INV, int Ns, int OFF, bool WAVE, int R, int... Rest>
 struct StockhamPass {
   static constexpr int E = N / TPL;
   static constexpr int NB = E / R;
@@ -434,33 +449,44 @@ struct StockhamPass {
       }
     }
     if constexpr (!LAST) {
-      __syncthreads();
+      lds_sync<WAVE>();
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = lds[LdsLayout<N>::at(t + TPL * q)];
-      __syncthreads();
-      StockhamPass<T, N, V, TPL, INV, Ns * R, (Ns > 1 ? OFF + NB * (R - 1) : OFF), Rest...>::run(v, lds, tw, t);
+      lds_sync<WAVE>();
+      StockhamPass<T, N, V, TPL, INV, Ns * R, (Ns > 1 ? OFF + NB * (R - 1) : OFF), WAVE, Rest...>::run(v, lds, tw,
+                                                                                                        t);
     }
   }
 };
 
-template <typename T, int N, int V, bool INV, typename RL>
+template <typename T, int N, int V, bool INV, bool WAVE, typename RL>
 struct LineFFTImpl;
-template <typename T, int N, int V, bool INV, int... Rs>
-struct LineFFTImpl<T, N, V, INV, Radices<Rs...>> {
+template <typename T, int N, int V, bool INV, bool WAVE, int... Rs>
+struct LineFFTImpl<T, N, V, INV, WAVE, Radices<Rs...>> {
   static constexpr int TPL = Plan<N, V>::TPL;
   static constexpr int E = Plan<N, V>::E;
   __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Twiddles<T, N, V>& tw, int t) {
-    StockhamPass<T, N, V, TPL, INV, 1, 0, Rs...>::run(v, lds, tw, t);
+    StockhamPass<T, N, V, TPL, INV, 1, 0, WAVE, Rs...>::run(v, lds, tw, t);
   }
 };
 
+// Lines of plan family V whose threads are consecutive (contiguous-line
+// passes) lie inside one wavefront when TPL divides 64.
+template <int N, int V>
+constexpr bool wave_local_lines() {
+  return 64 % Plan<N, V>::TPL == 0;
+}
+
 // Public entry: forward (INV=false) or unnormalised inverse transform of the
 // line held in v (stride-TPL layout), using the line's LDS slice `lds`.
-// Every thread of the workgroup must call it (it contains barriers).
-template <typename T, int N, bool INV, int V = 0>
+// Every thread of the workgroup must call it (it contains barriers).  WAVE:
+// the caller's lines are wave-local (consecutive threads, TPL dividing 64),
+// so the exchanges synchronise the wavefront only.
+template <typename T, int N, bool INV, int V = 0, bool WAVE = false>
 __device__ __forceinline__ void line_fft(Cx<T> (&v)[(Plan<N, V>::E)], Cx<T>* lds, const Twiddles<T, N, V>& tw,
                                          int t) {
-  LineFFTImpl<T, N, V, INV, typename Plan<N, V>::R>::run(v, lds, tw, t);
+  static_assert(!WAVE || 64 % Plan<N, V>::TPL == 0, "wave-local lines need TPL dividing 64");
+  LineFFTImpl<T, N, V, INV, WAVE, typename Plan<N, V>::R>::run(v, lds, tw, t);
 }
 
 }  // namespace kw

@@ -365,22 +365,23 @@ __device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)
 template <typename T, int N, bool CONTIG, bool TWO = false>
 __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int t_index) {
   constexpr int V = CONTIG ? 1 : 0;  // plan variant of contiguous x-line passes
+  constexpr bool WL = CONTIG && wave_local_lines<N, V>();   // lines inside one wavefront
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
   int lpw;
-  const LineCtx<T, N, V> L = make_ctx<T, N, CONTIG>(a, wg, lpw);
+  const LineCtx<T, N, V> L = make_ctx<T, N, CONTIG, V>(a, wg, lpw);
   Cx<T>* lds = lds_all + L.c * P;
   const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(a.tw, L.t);
 
   // --- grad p (staggered +) : rfft -> i k e^{+ikh/2} -> irfft
   Cx<T> v[E];
   L.load(a.p, v);
-  line_fft<T, N, false, V>(v, lds, tw, L.t);
+  line_fft<T, N, false, V, WL>(v, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dp[L.n(q)]);
-  line_fft<T, N, true, V>(v, lds, tw, L.t);
+  line_fft<T, N, true, V, WL>(v, lds, tw, L.t);
 
   // --- velocity update  u = pml_sg (pml_sg u - dt/rho0_sg dp)
   Cx<T> r0[E];
@@ -435,10 +436,10 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
   L.store(a.u, v);
 
   // --- div u (staggered -)
-  line_fft<T, N, false, V>(v, lds, tw, L.t);
+  line_fft<T, N, false, V, WL>(v, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dm[L.n(q)]);
-  line_fft<T, N, true, V>(v, lds, tw, L.t);
+  line_fft<T, N, true, V, WL>(v, lds, tw, L.t);
   if constexpr (!CONTIG) {
     // axisymmetric radial pass: du_r/dr + u_r / r, u_r as stored above (an L2 hit)
     if (a.inv_r != nullptr) {
@@ -576,10 +577,10 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
       Cx<T> z[E];
 #pragma unroll
       for (int q = 0; q < E; ++q) z[q] = s == 0 ? f[q] : g[TWO ? q : 0];
-      line_fft<T, N, false, V>(z, lds, tw, L.t);
+      line_fft<T, N, false, V, WL>(z, lds, tw, L.t);
 #pragma unroll
       for (int q = 0; q < E; ++q) lds[LdsLayout<N>::at(L.n(q))] = z[q];
-      __syncthreads();
+      lds_sync<WL>();
       Cx<T>* __restrict__ Sw = Sdst + 2LL * wg * lpw * a.nhp;
       const unsigned oa = (unsigned)(2 * L.c * a.nhp), ob = oa + a.nhp;
 #pragma unroll
@@ -595,7 +596,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
           if (L.vb) at32(Sw, ob + k) = B;
         }
       }
-      __syncthreads();
+      lds_sync<WL>();
     }
   }
 }
@@ -837,7 +838,7 @@ __device__ __forceinline__ void c2r_body(const C2RArgs<T>& a, int wg) {
     }
     z[q] = Cx<T>{A.x - B.y, A.y + B.x};
   }
-  line_fft<T, N, true, V>(z, lds, tw, t);
+  line_fft<T, N, true, V, wave_local_lines<N, V>()>(z, lds, tw, t);
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int n = t + TPL * q;
