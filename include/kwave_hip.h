@@ -161,12 +161,16 @@ int32_t kw_device_count(int32_t* count);
 int32_t kw_ctx_create(int32_t device, kw_ctx** out);
 int32_t kw_ctx_destroy(kw_ctx* ctx);
 
-/* 1 if an axis of length n runs on the line-FFT kernels, else 0. */
+/* How an axis of length n runs in kw_sim_create (the acoustic solver): 1 on a compiled
+ * line-FFT plan, 2 as chirp-z (Bluestein) lines on a power-of-two plan (any length up to
+ * 2048: 2 to 4x the line-FFT work, the same memory traffic), 0 not at all (n < 1, or longer
+ * than 2048 without a plan).  Batches, the adjoint, the elastic solver and the operator entry
+ * points take compiled lengths (1) only.  FFTW plans any length (fft_utils.jl:44-62). */
 int32_t kw_fft_length_supported(int64_t n);
 
 /*
  * 1 if a grid of ndim axes N[0..ndim-1] runs in `precision` (4 or 8), else 0:
- * every axis length has a line-FFT plan and no strided line spans more than
+ * every axis length runs (kw_fft_length_supported != 0) and no strided line spans more than
  * 4 GiB (the line passes use 32-bit per-lane byte offsets from a 64-bit
  * workgroup base; e.g. 1024^3 in Float32 and 1024^2 x 512 in Float64 exceed
  * it).  kw_sim_create / kw_elastic_create reject such grids with

@@ -32,6 +32,8 @@ SOURCES = [
     "kw_launch_misc.hip",
     "kw_launch_el_f32.hip",
     "kw_launch_el_f64.hip",
+    "kw_launch_bs_f32.hip",
+    "kw_launch_bs_f64.hip",
     "kw_api.hip",
     "kw_elastic.hip",
     "kw_ops.hip",

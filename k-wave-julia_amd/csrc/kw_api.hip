@@ -275,6 +275,8 @@ class Sim final : public SimBase {
     if (axisym_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient is not available for the axisymmetric solver");
     if (stokes_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient needs the heterogeneous equation-of-state branch");
     if (maps_f16_) raise(KW_ERR_UNSUPPORTED, "the c0 gradient needs maps in the kernel precision");
+    for (int a = 0; a < nd_; ++a)
+      if (bs_[a]) raise(KW_ERR_UNSUPPORTED, "the c0 gradient needs axis lengths with a compiled line-FFT plan");
     if (tr_ || (psrc_[0].any && p_dir_)) raise(KW_ERR_UNSUPPORTED, "time reversal / Dirichlet sources");
     for (int a = 0; a < nd_; ++a)
       if (usrc_[a].any && u_dir_) raise(KW_ERR_UNSUPPORTED, "Dirichlet velocity sources");
@@ -558,11 +560,15 @@ class Sim final : public SimBase {
     }
     for (int a = 0; a < nd_; ++a) {
       if (N_[a] < 1) raise(KW_ERR_INVALID, "grid size must be positive");
-      if (!size_supported(N_[a]))
+      // compiled plan, or a chirp-z (Bluestein) line on a power-of-two plan
+      bs_[a] = !size_supported(N_[a]);
+      M_[a] = bs_[a] ? bluestein_plan(N_[a]) : N_[a];
+      if (M_[a] == 0)
         raise(KW_ERR_UNSUPPORTED, "axis length " + std::to_string(N_[a]) +
-                                      " has no line-FFT plan (supported: 8,9,10,12,15,16,20,24,25,27,32,36,"
-                                      "40,45,48,64,72,75,80,81,96,100,125,128,135,144,160,192,200,225,243,256,"
-                                      "320,384,400,512,640,768,1024,2048,4096)");
+                                      " is longer than the largest chirp-z line (2048) and has no compiled plan");
+      if (bs_[a] && batch_mode_)
+        raise(KW_ERR_UNSUPPORTED, "axis length " + std::to_string(N_[a]) +
+                                      " has no compiled plan; batches take compiled lengths only");
       if (d.pml_size[a] < 0) raise(KW_ERR_INVALID, "pml_size must be >= 0");
       if (2LL * d.pml_size[a] > N_[a] + 0 && d.pml_size[a] > N_[a])
         raise(KW_ERR_INVALID, "pml_size larger than the grid");
@@ -623,7 +629,18 @@ class Sim final : public SimBase {
     // axis tables
     for (int a = 0; a < nd_; ++a) {
       AxisTables A = axis_tables(N_[a], d.d[a], d.pml_size[a], d.pml_alpha[a], d.dt);
-      upload(tw_[a], to_cx<T>(A.twr, A.twi));
+      if (bs_[a]) {
+        // the line FFT runs on the M-point plan; chirp tables carry length N
+        std::vector<double> wr, wi;
+        twiddles(M_[a], wr, wi);
+        upload(tw_[a], to_cx<T>(wr, wi));
+        const ChirpTables C = chirp_tables(N_[a], M_[a]);
+        upload(chc_[a], to_cx<T>(C.cr, C.ci));
+        upload(chf_[a], to_cx<T>(C.gfr, C.gfi));
+        upload(chb_[a], to_cx<T>(C.gbr, C.gbi));
+      } else {
+        upload(tw_[a], to_cx<T>(A.twr, A.twi));
+      }
       upload(Dp_[a], to_cx<T>(A.dpr, A.dpi));
       upload(Dm_[a], to_cx<T>(A.dmr, A.dmi));
       upload(pml_[a], to_t<T>(A.pml));
@@ -636,12 +653,13 @@ class Sim final : public SimBase {
 
     // launch geometry
 
-    cfg_[0] = pass_cfg(N_[0], true, N_[1] * N_[2], 0, TC_DIR_X);
-    if (nd_ > 1) cfg_[1] = pass_cfg(N_[1], false, N_[0], N_[2], TC_DIR_Y);
-    if (nd_ > 2) cfg_[2] = pass_cfg(N_[2], false, N_[0], N_[1], TC_DIR_Z);
-    if (nd_ > 1) spec_[0] = spec_cfg(N_[1], N_[2], TC_SPEC_Y);
-    if (nd_ > 2) spec_[1] = spec_cfg(N_[2], N_[1], TC_SPEC_Z);
-    c2r_ = pass_cfg(N_[0], true, N_[1] * N_[2], 0, TC_C2R);
+    // launch shapes follow the plan length (M_ = N_ unless the axis runs chirp-z lines)
+    cfg_[0] = pass_cfg(M_[0], true, N_[1] * N_[2], 0, TC_DIR_X, bs_[0]);
+    if (nd_ > 1) cfg_[1] = pass_cfg(M_[1], false, N_[0], N_[2], TC_DIR_Y);
+    if (nd_ > 2) cfg_[2] = pass_cfg(M_[2], false, N_[0], N_[1], TC_DIR_Z);
+    if (nd_ > 1) spec_[0] = spec_cfg(M_[1], N_[2], TC_SPEC_Y);
+    if (nd_ > 2) spec_[1] = spec_cfg(M_[2], N_[1], TC_SPEC_Z);
+    c2r_ = pass_cfg(M_[0], true, N_[1] * N_[2], 0, TC_C2R, bs_[0]);
 
     build_sources(d, het_c);
     build_sensor(d);
@@ -660,10 +678,11 @@ class Sim final : public SimBase {
     return h;
   }
 
-  PassCfg pass_cfg(long long n, bool contig, long long lines, long long outer, int cls) {
+  PassCfg pass_cfg(long long n, bool contig, long long lines, long long outer, int cls, bool bs = false) {
     PassCfg c;
     c.n = (int)n;
-    const int tpl = plan_tpl((int)n, cls == TC_C2R ? 2 : (contig ? 1 : 0));
+    // plan family: 0 strided, 1 contiguous, 2 complex-to-real, 4 contiguous chirp-z lines
+    const int tpl = plan_tpl((int)n, contig && bs ? 4 : (cls == TC_C2R ? 2 : (contig ? 1 : 0)));
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
     // strided lines: at least 16 line pairs per workgroup (128-byte rows in FP32)
     // when that fits 512 threads; contiguous lines: 256 threads
@@ -946,6 +965,8 @@ class Sim final : public SimBase {
       A.Nx = (int)N_[0];
       A.n_lines = (int)(N_[1] * N_[2]);
       A.inv_r = (axisym_ && a == 1) ? inv_r_.as<T>() : nullptr;
+      A.nl = (int)N_[a];
+      A.ch = chirp(a);
       const PassCfg& c = cfg_[a];
       A.x_blocks = (int)(((N_[0] + 1) / 2 + c.lpw - 1) / c.lpw);
       if (a == 1) {
@@ -1002,6 +1023,8 @@ class Sim final : public SimBase {
       Y.axis = 1;
       Y.Ny = (int)N_[1];
       Y.Nz = nd_ > 2 ? (int)N_[2] : 1;   // stacked firings: no octant index along the firing axis
+      Y.nl = (int)N_[1];
+      Y.ch = chirp(1);
       Y.MH1 = (int)NHP_;   // padded table row pitch
       Y.MH2 = hy;
     }
@@ -1021,6 +1044,8 @@ class Sim final : public SimBase {
       Z.axis = 2;
       Z.Ny = (int)N_[1];
       Z.Nz = (int)N_[2];
+      Z.nl = (int)N_[2];
+      Z.ch = chirp(2);
       Z.MH1 = (int)NHP_;
       Z.MH2 = hz;
     }
@@ -1033,6 +1058,8 @@ class Sim final : public SimBase {
     ca_.p = p_.as<T>();
     ca_.n_lines = (int)(N_[1] * N_[2]);
     ca_.nhp = (int)NHP_;
+    ca_.nl = (int)N_[0];
+    ca_.ch = chirp(0);
 
     // recording
     std::memset(&ra_, 0, sizeof(ra_));
@@ -1077,7 +1104,7 @@ class Sim final : public SimBase {
       PassArgs<T> A = pa_[a];
       A.mode = 1;
       A.eos = 0;
-      HIPC(launch_dir<T>(cfg_[a].n, a == 0, cfg_[a].grid, cfg_[a].block, cfg_[a].shmem, stream_, A));
+      HIPC(ldir(a, cfg_[a], A));
     }
     HIPC(hipStreamSynchronize(stream_));
   }
@@ -1122,9 +1149,11 @@ class Sim final : public SimBase {
         outer = a == 1 ? N_[2] : N_[1];
       }
       A.NH = (int)lines;
-      PassCfg c = line_cfg(n, lines, outer);
+      A.nl = (int)n;
+      A.ch = chirp(a);
+      PassCfg c = line_cfg(M_[a], lines, outer);
       A.x_blocks = (int)((lines + c.lpw - 1) / c.lpw);
-      HIPC(launch_spec<T>((int)n, c.grid, c.block, c.shmem, stream_, A));
+      HIPC(lspec(a, c, A));
       HIPC(hipStreamSynchronize(stream_));
     }
     const unsigned nr = std::min<unsigned>(nb, 1024u);
@@ -1133,6 +1162,31 @@ class Sim final : public SimBase {
     HIPC(launch_smooth_helpers<T>(2, nb, stream_, nullptr, z.as<Cx<T>>(), mx.as<unsigned long long>(), p0.as<T>(),
                                   nvox_));
     HIPC(hipStreamSynchronize(stream_));
+  }
+
+  // chirp tables of axis a (zero when its lines run a compiled plan)
+  Chirp<T> chirp(int a) {
+    Chirp<T> c{};
+    if (bs_[a]) {
+      c.n = (int)N_[a];
+      c.c = chc_[a].as<Cx<T>>();
+      c.gf = chf_[a].as<Cx<T>>();
+      c.gb = chb_[a].as<Cx<T>>();
+    }
+    return c;
+  }
+  // pass launches on axis a: the compiled-length kernels or their chirp-z form
+  hipError_t ldir(int a, const PassCfg& c, const PassArgs<T>& A) {
+    return bs_[a] ? launch_dir_bs<T>(c.n, a == 0, c.grid, c.block, c.shmem, stream_, A)
+                  : launch_dir<T>(c.n, a == 0, c.grid, c.block, c.shmem, stream_, A);
+  }
+  hipError_t lspec(int a, const PassCfg& c, const SpecArgs<T>& A) {
+    return bs_[a] ? launch_spec_bs<T>(c.n, c.grid, c.block, c.shmem, stream_, A)
+                  : launch_spec<T>(c.n, c.grid, c.block, c.shmem, stream_, A);
+  }
+  hipError_t lc2r(const PassCfg& c, const C2RArgs<T>& A) {
+    return bs_[0] ? launch_c2r_bs<T>(c.n, c.grid, c.block, c.shmem, stream_, A)
+                  : launch_c2r<T>(c.n, c.grid, c.block, c.shmem, stream_, A);
   }
 
   // launch geometry for complex lines of length n (lines x outer)
@@ -1166,27 +1220,27 @@ class Sim final : public SimBase {
     for (int a = nd_ - 1; a >= 1; --a) {
       pa_[a].t = t;
       launch(pr, a == 1 ? C_DIR_Y : C_DIR_Z,
-             [&] { HIPC(launch_dir<T>(cfg_[a].n, false, cfg_[a].grid, cfg_[a].block, cfg_[a].shmem, stream_, pa_[a])); });
+             [&] { HIPC(ldir(a, cfg_[a], pa_[a])); });
     }
     pa_[0].t = t;
     launch(pr, C_DIR_X,
-           [&] { HIPC(launch_dir<T>(cfg_[0].n, true, cfg_[0].grid, cfg_[0].block, cfg_[0].shmem, stream_, pa_[0])); });
+           [&] { HIPC(ldir(0, cfg_[0], pa_[0])); });
     if (nd_ == 3) {
       SpecArgs<T> Y = sa_[0];
       Y.mode = 0;
-      launch(pr, C_SPEC_Y_FWD, [&] { HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y)); });
+      launch(pr, C_SPEC_Y_FWD, [&] { HIPC(lspec(1, spec_[0], Y)); });
       SpecArgs<T> Z = sa_[1];
       Z.mode = 2;
-      launch(pr, C_SPEC_Z, [&] { HIPC(launch_spec<T>(spec_[1].n, spec_[1].grid, spec_[1].block, spec_[1].shmem, stream_, Z)); });
+      launch(pr, C_SPEC_Z, [&] { HIPC(lspec(2, spec_[1], Z)); });
       Y.mode = 1;
       Y.S2 = nullptr;
-      launch(pr, C_SPEC_Y_INV, [&] { HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y)); });
+      launch(pr, C_SPEC_Y_INV, [&] { HIPC(lspec(1, spec_[0], Y)); });
     } else if (nd_ == 2) {
       SpecArgs<T> Y = sa_[0];
       Y.mode = 2;
-      launch(pr, C_SPEC_Y, [&] { HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y)); });
+      launch(pr, C_SPEC_Y, [&] { HIPC(lspec(1, spec_[0], Y)); });
     }
-    launch(pr, C_C2R, [&] { HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, ca_)); });
+    launch(pr, C_C2R, [&] { HIPC(lc2r(c2r_, ca_)); });
     if (nsens_ > 0 && !cartesian_ && rec_ && !no_record_) {
       ra_.t = t;
       launch(pr, C_REC, [&] {
@@ -1259,6 +1313,9 @@ class Sim final : public SimBase {
 
   int dev_;
   bool batch_mode_ = false;
+  long long M_[3] = {1, 1, 1};   // line-FFT plan length per axis (N_, or the chirp-z plan)
+  bool bs_[3] = {false, false, false};   // axis runs chirp-z (Bluestein) lines
+  DevBuf chc_[3], chf_[3], chb_[3];       // chirp tables of those axes
   int nfire_ = 1;          // stacked firings (2-D), 1 otherwise
   long long plane_ = 0;    // voxels of one firing's grid
   hipStream_t stream_ = nullptr;
@@ -1737,7 +1794,7 @@ int32_t kw_ctx_destroy(kw_ctx* ctx) {
   return KW_OK;
 }
 
-int32_t kw_fft_length_supported(int64_t n) { return size_supported(n) ? 1 : 0; }
+int32_t kw_fft_length_supported(int64_t n) { return size_supported(n) ? 1 : (bluestein_plan(n) ? 2 : 0); }
 
 int32_t kw_desc_sizes(int64_t* sim_desc_bytes, int64_t* elastic_desc_bytes) {
   if (sim_desc_bytes) *sim_desc_bytes = (int64_t)sizeof(kw_sim_desc);
@@ -1749,7 +1806,7 @@ int32_t kw_grid_supported(int32_t ndim, const int64_t* N, int32_t precision) {
   if (ndim < 1 || ndim > 3 || N == nullptr || (precision != 4 && precision != 8)) return 0;
   long long n[3] = {1, 1, 1};
   for (int a = 0; a < ndim; ++a) {
-    if (!size_supported(N[a])) return 0;
+    if (!size_supported(N[a]) && !bluestein_plan(N[a])) return 0;
     n[a] = N[a];
   }
   return lane_offsets_fit(ndim, n, precision) ? 1 : 0;

@@ -316,6 +316,26 @@ struct Plan<256, 2> {
   using R = Radices<8, 8, 4>;
 };
 
+// V = 4: contiguous-line passes (x pass, complex-to-real) of chirp-z lines,
+// which carry the chirp and filter arrays on top of the field arrays: the long
+// plans use 8 elements per thread (the 16-element plans spill registers there).
+template <int N>
+struct Plan<N, 4> : Plan<N, 1> {};
+template <>
+struct Plan<1024, 4> {
+  static constexpr int N = 1024;
+  static constexpr int TPL = 128;
+  static constexpr int E = 8;
+  using R = Radices<8, 8, 4, 4>;
+};
+template <>
+struct Plan<2048, 4> {
+  static constexpr int N = 2048;
+  static constexpr int TPL = 256;
+  static constexpr int E = 8;
+  using R = Radices<8, 8, 8, 4>;
+};
+
 // LDS slice of one line: one complex of padding per 16 keeps the
 // stride-16/32 Stockham writes off a single bank group.  The 4096 plan is
 // left unpadded so a double-precision line still fits a 64 KiB workgroup.
@@ -487,6 +507,52 @@ __device__ __forceinline__ void line_fft(Cx<T> (&v)[(Plan<N, V>::E)], Cx<T>* lds
                                          int t) {
   static_assert(!WAVE || 64 % Plan<N, V>::TPL == 0, "wave-local lines need TPL dividing 64");
   LineFFTImpl<T, N, V, INV, WAVE, typename Plan<N, V>::R>::run(v, lds, tw, t);
+}
+
+// ---------------------------------------------------------------------------
+// Lines of any length n (Bluestein / chirp-z): the DFT of length n as a
+// circular convolution on a compiled power-of-two plan of length N >= 2n - 1.
+//   X_k = c_k sum_j (x_j c_j) d_(k-j),  c_j = exp(-i pi j^2 / n),  d = conj(c)
+//       = c_k IFFT_N( G . FFT_N(c . x) ),  G = FFT_N(g) / N
+// with g_m = d_m (0 <= m < n), g_(N-m) = d_m (0 < m < n), 0 elsewhere; the
+// unnormalised inverse uses conj(c) and the filter of c.  The tables are
+// generated in FP64 on the host.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Chirp {
+  int n;               // logical line length
+  const Cx<T>* c;      // c_j, j < n
+  const Cx<T>* gf;     // N values: forward filter spectrum, 1/N folded in
+  const Cx<T>* gb;     // N values: inverse filter spectrum, 1/N folded in
+};
+
+// Forward (INV=false) or unnormalised inverse DFT of a line of length ch.n held
+// in v (stride-TPL layout of the N-point plan; slots j >= n hold zero on exit
+// and are ignored on entry).  BS=false is the plain N-point line_fft.
+template <typename T, int N, bool INV, int V = 0, bool WAVE = false, bool BS = false>
+__device__ __forceinline__ void line_dft(Cx<T> (&v)[(Plan<N, V>::E)], Cx<T>* lds, const Twiddles<T, N, V>& tw, int t,
+                                         const Chirp<T>& ch) {
+  if constexpr (!BS) {
+    line_fft<T, N, INV, V, WAVE>(v, lds, tw, t);
+  } else {
+    constexpr int TPL = Plan<N, V>::TPL, E = Plan<N, V>::E;
+    const int n = ch.n;
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int j = t + TPL * q;
+      v[q] = j < n ? (INV ? cmulc(v[q], ch.c[j]) : cmul(v[q], ch.c[j])) : Cx<T>{0, 0};
+    }
+    line_fft<T, N, false, V, WAVE>(v, lds, tw, t);
+    const Cx<T>* g = INV ? ch.gb : ch.gf;
+#pragma unroll
+    for (int q = 0; q < E; ++q) v[q] = cmul(v[q], g[t + TPL * q]);
+    line_fft<T, N, true, V, WAVE>(v, lds, tw, t);
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int j = t + TPL * q;
+      v[q] = j < n ? (INV ? cmulc(v[q], ch.c[j]) : cmul(v[q], ch.c[j])) : Cx<T>{0, 0};
+    }
+  }
 }
 
 }  // namespace kw

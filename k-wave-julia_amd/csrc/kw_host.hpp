@@ -128,6 +128,77 @@ std::vector<T> to_t(const std::vector<double>& v) {
   return o;
 }
 
+// Chirp-z (Bluestein) tables of a line of length n on the power-of-two plan m
+// (kw_fft.hpp line_dft), FP64: c_j = exp(-i pi (j^2 mod 2n) / n) for j < n;
+// gf = FFT_m(g) / m with g_k = conj(c_k) for 0 <= k < n and g_(m-k) = conj(c_k)
+// for 0 < k < n (zero elsewhere); gb the same for the filter of c (inverse
+// transform).
+struct ChirpTables {
+  std::vector<double> cr, ci, gfr, gfi, gbr, gbi;
+};
+// In-place forward DFT of length m (a power of two), iterative radix 2.
+inline void host_fft_pow2(std::vector<double>& re, std::vector<double>& im) {
+  const size_t m = re.size();
+  for (size_t i = 1, j = 0; i < m; ++i) {
+    size_t bit = m >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      std::swap(re[i], re[j]);
+      std::swap(im[i], im[j]);
+    }
+  }
+  for (size_t len = 2; len <= m; len <<= 1) {
+    for (size_t i = 0; i < m; i += len)
+      for (size_t k = 0; k < len / 2; ++k) {
+        const double a = -2.0 * M_PI * (double)k / (double)len;
+        const double wr = std::cos(a), wi = std::sin(a);
+        const size_t u = i + k, v = i + k + len / 2;
+        const double xr = re[v] * wr - im[v] * wi, xi = re[v] * wi + im[v] * wr;
+        re[v] = re[u] - xr;
+        im[v] = im[u] - xi;
+        re[u] += xr;
+        im[u] += xi;
+      }
+  }
+}
+inline ChirpTables chirp_tables(long long n, long long m) {
+  ChirpTables C;
+  C.cr.resize(n);
+  C.ci.resize(n);
+  for (long long j = 0; j < n; ++j) {
+    const long long q = (j * j) % (2 * n);
+    const double a = -M_PI * (double)q / (double)n;
+    C.cr[j] = std::cos(a);
+    C.ci[j] = std::sin(a);
+  }
+  for (int dir = 0; dir < 2; ++dir) {
+    std::vector<double> gr(m, 0.0), gi(m, 0.0);
+    const double sg = dir == 0 ? -1.0 : 1.0;   // forward filter conj(c), inverse filter c
+    for (long long k = 0; k < n; ++k) {
+      gr[k] = C.cr[k];
+      gi[k] = sg * C.ci[k];
+      if (k > 0) {
+        gr[m - k] = C.cr[k];
+        gi[m - k] = sg * C.ci[k];
+      }
+    }
+    host_fft_pow2(gr, gi);
+    for (long long k = 0; k < m; ++k) {
+      gr[k] /= (double)m;
+      gi[k] /= (double)m;
+    }
+    if (dir == 0) {
+      C.gfr = gr;
+      C.gfi = gi;
+    } else {
+      C.gbr = gr;
+      C.gbi = gi;
+    }
+  }
+  return C;
+}
+
 struct AxisTables {
   std::vector<double> twr, twi, dpr, dpi, dmr, dmi, pml, pml_sg;
 };

@@ -106,6 +106,9 @@ struct PassArgs {
   int nhp;             // half-spectrum row pitch (Nx/2+1 rounded up to 16 complex)
   T* rho_sum_out;      // if set, the x pass also stores sum_d rho_d (adjoint recompute)
   const T* inv_r;      // axisymmetric radial pass: 1/r along the line (else null)
+  // lines of any length (Bluestein kernels): line length and chirp tables
+  int nl;
+  Chirp<T> ch;
 };
 
 // ---------------------------------------------------------------------------
@@ -141,11 +144,14 @@ __device__ __forceinline__ U& at32(U* base, unsigned i) {
 
 // Addresses are a per-workgroup 64-bit base (uniform) plus 32-bit lane
 // offsets, so every load is a scalar-base + vector-offset access.
-template <typename T, int N, int V = 0>
+// BS: the line has a runtime length nl < N (Bluestein lines on the N-point
+// plan); register slots j >= nl hold no element and are never loaded/stored.
+template <typename T, int N, int V = 0, bool BS = false>
 struct LineCtx {
   static constexpr int TPL = Plan<N, V>::TPL;
   static constexpr int E = Plan<N, V>::E;
   using V2 = typename std::conditional<sizeof(T) == 4, float2, double2>::type;
+  int nl = N;           // line length (elements)
   int t, c;
   bool va, vb;          // real line a / b valid
   bool vec = false;     // strided pairs are adjacent and aligned: one 2-element access per pair
@@ -155,10 +161,16 @@ struct LineCtx {
   __device__ __forceinline__ unsigned ia(int q) const { return oa + sa * (unsigned)(t + TPL * q); }
   __device__ __forceinline__ unsigned ib(int q) const { return ob + sa * (unsigned)(t + TPL * q); }
   __device__ __forceinline__ int n(int q) const { return t + TPL * q; }
+  __device__ __forceinline__ bool ok(int q) const { return !BS || t + TPL * q < nl; }   // slot holds an element
   __device__ __forceinline__ unsigned nx(int q) const {  // offset of element n+1 (periodic)
     int nn = t + TPL * q + 1;
-    nn = (nn == N) ? 0 : nn;
+    nn = (nn == (BS ? nl : N)) ? 0 : nn;
     return sa * (unsigned)nn;
+  }
+  // entry n(q) of a per-line table of nl values (0 for slots without an element)
+  template <typename U>
+  __device__ __forceinline__ U tab(const U* p, int q) const {
+    return ok(q) ? p[t + TPL * q] : U{};
   }
 
   // `fill` is what an absent line of the pair holds (1 for divisors like rho0)
@@ -168,15 +180,15 @@ struct LineCtx {
       const V2* __restrict__ f2 = reinterpret_cast<const V2*>(f);
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        const V2 r = va ? at32(f2, ia(q) >> 1) : V2{fill, fill};
+        const V2 r = (va && ok(q)) ? at32(f2, ia(q) >> 1) : V2{fill, fill};
         v[q] = Cx<T>{r.x, r.y};
       }
       return;
     }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      v[q].x = va ? at32(f, ia(q)) : fill;
-      v[q].y = vb ? at32(f, ib(q)) : fill;
+      v[q].x = (va && ok(q)) ? at32(f, ia(q)) : fill;
+      v[q].y = (vb && ok(q)) ? at32(f, ib(q)) : fill;
     }
   }
   __device__ __forceinline__ void store(T* __restrict__ f0, const Cx<T> (&v)[E]) const {
@@ -185,13 +197,13 @@ struct LineCtx {
       V2* __restrict__ f2 = reinterpret_cast<V2*>(f);
 #pragma unroll
       for (int q = 0; q < E; ++q)
-        if (va) at32(f2, ia(q) >> 1) = V2{v[q].x, v[q].y};
+        if (va && ok(q)) at32(f2, ia(q) >> 1) = V2{v[q].x, v[q].y};
       return;
     }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      if (va) at32(f, ia(q)) = v[q].x;
-      if (vb) at32(f, ib(q)) = v[q].y;
+      if (va && ok(q)) at32(f, ia(q)) = v[q].x;
+      if (vb && ok(q)) at32(f, ib(q)) = v[q].y;
     }
   }
   // pair of map values at element offset i (even) of the workgroup base
@@ -216,7 +228,7 @@ struct LineCtx {
         load_half_pairs(f, v, off, fill, 0);
       } else {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = va ? map_pair(f, ia(q), false, off) : Cx<T>{fill, fill};
+        for (int q = 0; q < E; ++q) v[q] = (va && ok(q)) ? map_pair(f, ia(q), false, off) : Cx<T>{fill, fill};
       }
       return;
     }
@@ -225,8 +237,8 @@ struct LineCtx {
     } else {
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        v[q].x = va ? map_at(f, ia(q), false, off) : fill;
-        v[q].y = vb ? map_at(f, ib(q), false, off) : fill;
+        v[q].x = (va && ok(q)) ? map_at(f, ia(q), false, off) : fill;
+        v[q].y = (vb && ok(q)) ? map_at(f, ib(q), false, off) : fill;
       }
     }
   }
@@ -237,13 +249,13 @@ struct LineCtx {
     unsigned r[E] = {};
     if (va) {
 #pragma unroll
-      for (int q = 0; q < E; ++q) r[q] = at32(h2, (next ? oa + nx(q) : ia(q)) >> 1);
+      for (int q = 0; q < E; ++q) r[q] = ok(q) ? at32(h2, (next ? oa + nx(q) : ia(q)) >> 1) : 0u;
     }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const __half lo = __ushort_as_half((unsigned short)(r[q] & 0xFFFFu));
       const __half hi = __ushort_as_half((unsigned short)(r[q] >> 16));
-      v[q] = va ? Cx<T>{off + (T)__half2float(lo), off + (T)__half2float(hi)} : Cx<T>{fill, fill};
+      v[q] = (va && ok(q)) ? Cx<T>{off + (T)__half2float(lo), off + (T)__half2float(hi)} : Cx<T>{fill, fill};
     }
   }
   // FP16 map elements of lines a and b: all raw 16-bit loads of a line are
@@ -254,16 +266,16 @@ struct LineCtx {
     unsigned short ra[E] = {}, rb[E] = {};
     if (va) {
 #pragma unroll
-      for (int q = 0; q < E; ++q) ra[q] = at32(h, next ? oa + nx(q) : ia(q));
+      for (int q = 0; q < E; ++q) ra[q] = ok(q) ? at32(h, next ? oa + nx(q) : ia(q)) : (unsigned short)0;
     }
     if (vb) {
 #pragma unroll
-      for (int q = 0; q < E; ++q) rb[q] = at32(h, next ? ob + nx(q) : ib(q));
+      for (int q = 0; q < E; ++q) rb[q] = ok(q) ? at32(h, next ? ob + nx(q) : ib(q)) : (unsigned short)0;
     }
 #pragma unroll
     for (int q = 0; q < E; ++q) {
-      v[q].x = va ? off + (T)__half2float(__ushort_as_half(ra[q])) : fill;
-      v[q].y = vb ? off + (T)__half2float(__ushort_as_half(rb[q])) : fill;
+      v[q].x = (va && ok(q)) ? off + (T)__half2float(__ushort_as_half(ra[q])) : fill;
+      v[q].y = (vb && ok(q)) ? off + (T)__half2float(__ushort_as_half(rb[q])) : fill;
     }
   }
   __device__ __forceinline__ void load_map_next(const void* f, Cx<T> (&v)[E], bool f16, T off, T fill) const {
@@ -272,7 +284,7 @@ struct LineCtx {
         load_half_pairs(f, v, off, fill, 1);
       } else {
 #pragma unroll
-        for (int q = 0; q < E; ++q) v[q] = va ? map_pair(f, oa + nx(q), false, off) : Cx<T>{fill, fill};
+        for (int q = 0; q < E; ++q) v[q] = (va && ok(q)) ? map_pair(f, oa + nx(q), false, off) : Cx<T>{fill, fill};
       }
       return;
     }
@@ -281,19 +293,20 @@ struct LineCtx {
     } else {
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        v[q].x = va ? map_at(f, oa + nx(q), false, off) : fill;
-        v[q].y = vb ? map_at(f, ob + nx(q), false, off) : fill;
+        v[q].x = (va && ok(q)) ? map_at(f, oa + nx(q), false, off) : fill;
+        v[q].y = (vb && ok(q)) ? map_at(f, ob + nx(q), false, off) : fill;
       }
     }
   }
 };
 
 // VV selects the plan family (default: 1 for contiguous x lines, 0 for strided).
-template <typename T, int N, bool CONTIG, int VV = (CONTIG ? 1 : 0)>
-__device__ __forceinline__ LineCtx<T, N, VV> make_ctx(const PassArgs<T>& a, int wg, int& lpw_out) {
+template <typename T, int N, bool CONTIG, int VV = (CONTIG ? 1 : 0), bool BS = false>
+__device__ __forceinline__ LineCtx<T, N, VV, BS> make_ctx(const PassArgs<T>& a, int wg, int& lpw_out) {
   constexpr int V = VV;
   constexpr int TPL = Plan<N, V>::TPL;
-  LineCtx<T, N, V> L;
+  LineCtx<T, N, V, BS> L;
+  L.nl = BS ? a.nl : N;
   const int tid = threadIdx.x;
   const int lpw = blockDim.x / TPL;
   lpw_out = lpw;
@@ -303,9 +316,9 @@ __device__ __forceinline__ LineCtx<T, N, VV> make_ctx(const PassArgs<T>& a, int 
     const long long pi = (long long)wg * lpw + L.c;
     L.va = 2 * pi < a.n_lines;
     L.vb = 2 * pi + 1 < a.n_lines;
-    L.wgb = 2LL * wg * lpw * N;
-    L.oa = (unsigned)(2 * L.c * N);
-    L.ob = L.oa + N;
+    L.wgb = 2LL * wg * lpw * L.nl;
+    L.oa = (unsigned)(2 * L.c * L.nl);
+    L.ob = L.oa + L.nl;
     L.sa = 1;
   } else {
     L.c = tid % lpw;
@@ -328,9 +341,9 @@ __device__ __forceinline__ LineCtx<T, N, VV> make_ctx(const PassArgs<T>& a, int 
 
 // Apply the workgroup's source entries to the pair-line registers through LDS.
 // pos packs (c << 21) | (part << 20) | n.
-template <typename T, int N, int V>
+template <typename T, int N, int V, bool BS = false>
 __device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)[(Plan<N, V>::E)], Cx<T>* lds_all,
-                                       const LineCtx<T, N, V>& L, int wg) {
+                                       const LineCtx<T, N, V, BS>& L, int wg) {
   if (s.off == nullptr) return;
   const int e0 = s.off[wg], e1 = s.off[wg + 1];
   if (e0 >= e1) return;  // uniform across the workgroup
@@ -362,26 +375,26 @@ __device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)
 // TWO (x pass only): the variant that can also form the second spectrum of
 // the homogeneous :stokes branch; without it the kernel carries no registers
 // for that path.
-template <typename T, int N, bool CONTIG, bool TWO = false>
+template <typename T, int N, bool CONTIG, bool TWO = false, bool BS = false>
 __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int t_index) {
-  constexpr int V = CONTIG ? 1 : 0;  // plan variant of contiguous x-line passes
+  constexpr int V = CONTIG ? (BS ? 4 : 1) : 0;  // plan variant of contiguous x-line passes
   constexpr bool WL = CONTIG && wave_local_lines<N, V>();   // lines inside one wavefront
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
   int lpw;
-  const LineCtx<T, N, V> L = make_ctx<T, N, CONTIG, V>(a, wg, lpw);
+  const LineCtx<T, N, V, BS> L = make_ctx<T, N, CONTIG, V, BS>(a, wg, lpw);
   Cx<T>* lds = lds_all + L.c * P;
   const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(a.tw, L.t);
 
   // --- grad p (staggered +) : rfft -> i k e^{+ikh/2} -> irfft
   Cx<T> v[E];
   L.load(a.p, v);
-  line_fft<T, N, false, V, WL>(v, lds, tw, L.t);
+  line_dft<T, N, false, V, WL, BS>(v, lds, tw, L.t, a.ch);
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dp[L.n(q)]);
-  line_fft<T, N, true, V, WL>(v, lds, tw, L.t);
+  for (int q = 0; q < E; ++q) v[q] = cmul(v[q], L.tab(a.Dp, q));
+  line_dft<T, N, true, V, WL, BS>(v, lds, tw, L.t, a.ch);
 
   // --- velocity update  u = pml_sg (pml_sg u - dt/rho0_sg dp)
   Cx<T> r0[E];
@@ -406,7 +419,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
       L.load(a.u, w);
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        const T s = a.pml_sg[L.n(q)];
+        const T s = L.tab(a.pml_sg, q);
         v[q].x = s * (s * w[q].x - fdiv(a.dt, r1[q].x) * v[q].x);
         v[q].y = s * (s * w[q].y - fdiv(a.dt, r1[q].y) * v[q].y);
       }
@@ -422,7 +435,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
       const T f = a.dt / a.rho0_s;
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        const T s = a.pml_sg[L.n(q)];
+        const T s = L.tab(a.pml_sg, q);
         v[q].x = s * (s * w[q].x - f * v[q].x);
         v[q].y = s * (s * w[q].y - f * v[q].y);
       }
@@ -432,14 +445,14 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
     L.store(a.u, v);
     return;
   }
-  inject<T, N, V>(a.usrc, t_index, v, lds_all, L, wg);
+  inject<T, N, V, BS>(a.usrc, t_index, v, lds_all, L, wg);
   L.store(a.u, v);
 
   // --- div u (staggered -)
-  line_fft<T, N, false, V, WL>(v, lds, tw, L.t);
+  line_dft<T, N, false, V, WL, BS>(v, lds, tw, L.t, a.ch);
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dm[L.n(q)]);
-  line_fft<T, N, true, V, WL>(v, lds, tw, L.t);
+  for (int q = 0; q < E; ++q) v[q] = cmul(v[q], L.tab(a.Dm, q));
+  line_dft<T, N, true, V, WL, BS>(v, lds, tw, L.t, a.ch);
   if constexpr (!CONTIG) {
     // axisymmetric radial pass: du_r/dr + u_r / r, u_r as stored above (an L2 hit)
     if (a.inv_r != nullptr) {
@@ -447,7 +460,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
       L.load(a.u, ur);
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        const T ir = a.inv_r[L.n(q)];
+        const T ir = L.tab(a.inv_r, q);
         v[q].x += ur[q].x * ir;
         v[q].y += ur[q].y * ir;
       }
@@ -465,7 +478,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
     if (a.rho0 != nullptr) {
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        const T s = a.pml[L.n(q)];
+        const T s = L.tab(a.pml, q);
         w[q].x = s * (s * w[q].x - a.dt * r0[q].x * v[q].x);
         w[q].y = s * (s * w[q].y - a.dt * r0[q].y * v[q].y);
       }
@@ -473,7 +486,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
       const T f = a.dt * a.rho0_s;
 #pragma unroll
       for (int q = 0; q < E; ++q) {
-        const T s = a.pml[L.n(q)];
+        const T s = L.tab(a.pml, q);
         w[q].x = s * (s * w[q].x - f * v[q].x);
         w[q].y = s * (s * w[q].y - f * v[q].y);
       }
@@ -481,7 +494,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = w[q];
   }
-  inject<T, N, V>(a.psrc, t_index, v, lds_all, L, wg);
+  inject<T, N, V, BS>(a.psrc, t_index, v, lds_all, L, wg);
   L.store(a.rho, v);
 
   if constexpr (CONTIG) {
@@ -569,7 +582,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
     }
     // r2c of the pair: Z = FFT(f_a + i f_b); A_k = (Z_k + conj Z_{N-k})/2,
     // B_k = (Z_k - conj Z_{N-k})/(2i), k = 0..N/2.
-    constexpr int NH = N / 2 + 1;
+    const int nhl = BS ? L.nl / 2 + 1 : N / 2 + 1;   // half-spectrum length of the line
 #pragma unroll 1
     for (int s = 0; s < (TWO ? 2 : 1); ++s) {
       if (s == 1 && !two) break;
@@ -577,7 +590,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
       Cx<T> z[E];
 #pragma unroll
       for (int q = 0; q < E; ++q) z[q] = s == 0 ? f[q] : g[TWO ? q : 0];
-      line_fft<T, N, false, V, WL>(z, lds, tw, L.t);
+      line_dft<T, N, false, V, WL, BS>(z, lds, tw, L.t, a.ch);
 #pragma unroll
       for (int q = 0; q < E; ++q) lds[LdsLayout<N>::at(L.n(q))] = z[q];
       lds_sync<WL>();
@@ -586,8 +599,8 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int k = L.n(q);
-        if (k < NH) {
-          const int km = (k == 0) ? 0 : N - k;
+        if (k < nhl) {
+          const int km = (k == 0) ? 0 : (BS ? L.nl : N) - k;
           const Cx<T> zk = z[q];
           const Cx<T> zm = lds[LdsLayout<N>::at(km)];
           const Cx<T> A = {(zk.x + zm.x) * (T)0.5, (zk.y - zm.y) * (T)0.5};
@@ -601,9 +614,9 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
   }
 }
 
-template <typename T, int N, bool CONTIG, bool TWO = false>
+template <typename T, int N, bool CONTIG, bool TWO = false, bool BS = false>
 __global__ void __launch_bounds__(512) k_dir_pass(PassArgs<T> a) {
-  dir_pass_body<T, N, CONTIG, TWO>(a, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), a.t);
+  dir_pass_body<T, N, CONTIG, TWO, BS>(a, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), a.t);
 }
 
 // Batched form: one launch covers the workgroups of many simulations of the
@@ -639,13 +652,16 @@ struct SpecArgs {
   int axis;           // 1 = line along y, 2 = line along z
   int Ny, Nz;
   int MH1, MH2;       // table row pitch (>= Nx/2+1) and the octant extent of the line axis
+  // lines of any length (Bluestein kernels): line length and chirp tables
+  int nl;
+  Chirp<T> ch;
 };
 
 // MODE: 0 forward (both spectra when S2 is set), 1 inverse, 2 fwd*M1*inv,
 // 4 fwd*(M1 S1 + M2 S2)*inv (:stokes), 3 fwd*W*inv, 5 fwd then S1 = inv(M1 .),
 // S2 = inv(M2 .) (the adjoint's K and dK/dc_ref applied to one spectrum).  Specialising on the mode
 // keeps the plain transforms at a small register footprint.
-template <typename T, int N, int MODE>
+template <typename T, int N, int MODE, bool BS = false>
 __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
   constexpr int TPL = Plan<N>::TPL;
   constexpr int E = Plan<N>::E;
@@ -663,37 +679,41 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
   const long long wgb = (long long)(xb * lpw) * a.Sc + a.So * o;   // uniform
   const unsigned lo = (unsigned)(c * a.Sc), sa = (unsigned)a.Sa;
   auto off = [&](int q) { return lo + sa * (unsigned)(t + TPL * q); };
+  const int nl = BS ? a.nl : N;                                    // line length
+  auto ok = [&](int q) { return !BS || t + TPL * q < nl; };      // slot holds an element
   Cx<T>* __restrict__ S1 = a.S1 + wgb;
 
   Cx<T> v[E];
 #pragma unroll
-  for (int q = 0; q < E; ++q) v[q] = valid ? at32(S1, off(q)) : Cx<T>{0, 0};
+  for (int q = 0; q < E; ++q) v[q] = (valid && ok(q)) ? at32(S1, off(q)) : Cx<T>{0, 0};
   if constexpr (MODE == 1) {
-    line_fft<T, N, true>(v, lds, tw, t);
+    line_dft<T, N, true, 0, false, BS>(v, lds, tw, t, a.ch);
   } else if constexpr (MODE == 3) {
-    line_fft<T, N, false>(v, lds, tw, t);
+    line_dft<T, N, false, 0, false, BS>(v, lds, tw, t, a.ch);
 #pragma unroll
-    for (int q = 0; q < E; ++q) v[q] = cscale(v[q], a.W[t + TPL * q]);
-    line_fft<T, N, true>(v, lds, tw, t);
+    for (int q = 0; q < E; ++q) v[q] = cscale(v[q], ok(q) ? a.W[t + TPL * q] : (T)0);
+    line_dft<T, N, true, 0, false, BS>(v, lds, tw, t, a.ch);
   } else if constexpr (MODE == 0) {
-    line_fft<T, N, false>(v, lds, tw, t);
+    line_dft<T, N, false, 0, false, BS>(v, lds, tw, t, a.ch);
     if (a.S2) {
       Cx<T>* __restrict__ S2 = a.S2 + wgb;
       if (valid) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) at32(S1, off(q)) = v[q];
+        for (int q = 0; q < E; ++q)
+          if (ok(q)) at32(S1, off(q)) = v[q];
       }
 #pragma unroll
-      for (int q = 0; q < E; ++q) v[q] = valid ? at32(S2, off(q)) : Cx<T>{0, 0};
-      line_fft<T, N, false>(v, lds, tw, t);
+      for (int q = 0; q < E; ++q) v[q] = (valid && ok(q)) ? at32(S2, off(q)) : Cx<T>{0, 0};
+      line_dft<T, N, false, 0, false, BS>(v, lds, tw, t, a.ch);
       if (valid) {
 #pragma unroll
-        for (int q = 0; q < E; ++q) at32(S2, off(q)) = v[q];
+        for (int q = 0; q < E; ++q)
+          if (ok(q)) at32(S2, off(q)) = v[q];
       }
       return;
     }
   } else if constexpr (MODE == 5) {
-    line_fft<T, N, false>(v, lds, tw, t);
+    line_dft<T, N, false, 0, false, BS>(v, lds, tw, t, a.ch);
     int io = o;
     if (a.axis == 1)
       io = a.Nz > 1 ? (io > a.Nz - io ? a.Nz - io : io) : 0;   // 2-D: outer index = stacked firing
@@ -703,22 +723,23 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       int ik = t + TPL * q;
-      ik = (ik > N - ik) ? N - ik : ik;
+      ik = (ik > nl - ik) ? nl - ik : ik;
       const unsigned mi = (unsigned)(kx + a.MH1 * (ik + a.MH2 * io));
-      const T m1 = valid ? at32(a.M1, mi) : (T)0;
-      const T m2 = valid ? at32(a.M2, mi) : (T)0;
+      const T m1 = (valid && ok(q)) ? at32(a.M1, mi) : (T)0;
+      const T m2 = (valid && ok(q)) ? at32(a.M2, mi) : (T)0;
       w[q] = cscale(v[q], m2);
       v[q] = cscale(v[q], m1);
     }
-    line_fft<T, N, true>(w, lds, tw, t);
+    line_dft<T, N, true, 0, false, BS>(w, lds, tw, t, a.ch);
     if (valid) {
       Cx<T>* __restrict__ S2 = a.S2 + wgb;
 #pragma unroll
-      for (int q = 0; q < E; ++q) at32(S2, off(q)) = w[q];
+      for (int q = 0; q < E; ++q)
+        if (ok(q)) at32(S2, off(q)) = w[q];
     }
-    line_fft<T, N, true>(v, lds, tw, t);
+    line_dft<T, N, true, 0, false, BS>(v, lds, tw, t, a.ch);
   } else {
-    line_fft<T, N, false>(v, lds, tw, t);
+    line_dft<T, N, false, 0, false, BS>(v, lds, tw, t, a.ch);
     Cx<T> w[MODE == 4 ? E : 1];
     // In a batch, a MODE 4 launch also covers simulations without a second
     // spectrum (S2 and M2 null): those take w = 0 and never touch S2/M2.
@@ -727,12 +748,12 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
       if (has2) {
         const Cx<T>* __restrict__ S2 = a.S2 + wgb;
 #pragma unroll
-        for (int q = 0; q < E; ++q) w[q] = valid ? at32(S2, off(q)) : Cx<T>{0, 0};
+        for (int q = 0; q < E; ++q) w[q] = (valid && ok(q)) ? at32(S2, off(q)) : Cx<T>{0, 0};
       } else {
 #pragma unroll
         for (int q = 0; q < E; ++q) w[q] = Cx<T>{0, 0};
       }
-      line_fft<T, N, false>(w, lds, tw, t);
+      line_dft<T, N, false, 0, false, BS>(w, lds, tw, t, a.ch);
     }
     // octant index |i| of the outer coordinate (z for y lines, y for z lines)
     int io = o;
@@ -743,27 +764,28 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       int ik = t + TPL * q;
-      ik = (ik > N - ik) ? N - ik : ik;
+      ik = (ik > nl - ik) ? nl - ik : ik;
       const unsigned mi = (unsigned)(kx + a.MH1 * (ik + a.MH2 * io));
-      const T m1 = valid ? at32(a.M1, mi) : (T)0;
+      const T m1 = (valid && ok(q)) ? at32(a.M1, mi) : (T)0;
       Cx<T> r = cscale(v[q], m1);
       if constexpr (MODE == 4) {
-        const T m2 = (valid && has2) ? at32(a.M2, mi) : (T)0;
+        const T m2 = (valid && has2 && ok(q)) ? at32(a.M2, mi) : (T)0;
         r = cadd(r, cscale(w[q], m2));
       }
       v[q] = r;
     }
-    line_fft<T, N, true>(v, lds, tw, t);
+    line_dft<T, N, true, 0, false, BS>(v, lds, tw, t, a.ch);
   }
   if (valid) {
 #pragma unroll
-    for (int q = 0; q < E; ++q) at32(S1, off(q)) = v[q];
+    for (int q = 0; q < E; ++q)
+      if (ok(q)) at32(S1, off(q)) = v[q];
   }
 }
 
-template <typename T, int N, int MODE>
+template <typename T, int N, int MODE, bool BS = false>
 __global__ void __launch_bounds__(512) k_spec_pass(SpecArgs<T> a) {
-  spec_pass_body<T, N, MODE>(a, strided_wg<T, N>(blockIdx.x));
+  spec_pass_body<T, N, MODE, BS>(a, strided_wg<T, N>(blockIdx.x));
 }
 
 template <typename T, int N, int MODE>
@@ -789,15 +811,19 @@ struct C2RArgs {
   T* p;
   int n_lines;
   int nhp;           // half-spectrum row pitch
+  // lines of any length (Bluestein kernels): line length and chirp tables
+  int nl;
+  Chirp<T> ch;
 };
 
-template <typename T, int N>
+template <typename T, int N, bool BS = false>
 __device__ __forceinline__ void c2r_body(const C2RArgs<T>& a, int wg) {
-  constexpr int V = 2;  // contiguous x lines, complex-to-real plan family
+  constexpr int V = BS ? 4 : 2;  // contiguous x lines, complex-to-real plan family (4: chirp-z lines)
   constexpr int TPL = Plan<N, V>::TPL;
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
-  constexpr int NH = N / 2 + 1;
+  const int nl = BS ? a.nl : N;          // line length
+  const int NH = nl / 2 + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
   const int lpw = blockDim.x / TPL;
@@ -811,24 +837,25 @@ __device__ __forceinline__ void c2r_body(const C2RArgs<T>& a, int wg) {
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int k = t + TPL * q;
+    const bool in = !BS || k < nl;
     const bool mir = k >= NH;
-    const int kk = mir ? N - k : k;
-    Cx<T> A = va ? a.S1[la * a.nhp + kk] : Cx<T>{0, 0};
-    Cx<T> B = vb ? a.S1[lb * a.nhp + kk] : Cx<T>{0, 0};
+    const int kk = !in ? 0 : (mir ? nl - k : k);
+    Cx<T> A = (va && in) ? a.S1[la * a.nhp + kk] : Cx<T>{0, 0};
+    Cx<T> B = (vb && in) ? a.S1[lb * a.nhp + kk] : Cx<T>{0, 0};
     if (a.M1) {
       const T m1 = a.M1[kk];
       A = cscale(A, m1);
       B = cscale(B, m1);
       if (a.S2) {
         const T m2 = a.M2[kk];
-        const Cx<T> A2 = va ? a.S2[la * a.nhp + kk] : Cx<T>{0, 0};
-        const Cx<T> B2 = vb ? a.S2[lb * a.nhp + kk] : Cx<T>{0, 0};
+        const Cx<T> A2 = (va && in) ? a.S2[la * a.nhp + kk] : Cx<T>{0, 0};
+        const Cx<T> B2 = (vb && in) ? a.S2[lb * a.nhp + kk] : Cx<T>{0, 0};
         A = cadd(A, cscale(A2, m2));
         B = cadd(B, cscale(B2, m2));
       }
     }
     // c2r semantics: imaginary parts of the DC and (even-N) Nyquist bins are ignored
-    if (kk == 0 || 2 * kk == N) {
+    if (kk == 0 || 2 * kk == nl) {
       A.y = 0;
       B.y = 0;
     }
@@ -838,18 +865,19 @@ __device__ __forceinline__ void c2r_body(const C2RArgs<T>& a, int wg) {
     }
     z[q] = Cx<T>{A.x - B.y, A.y + B.x};
   }
-  line_fft<T, N, true, V, wave_local_lines<N, V>()>(z, lds, tw, t);
+  line_dft<T, N, true, V, wave_local_lines<N, V>(), BS>(z, lds, tw, t, a.ch);
 #pragma unroll
   for (int q = 0; q < E; ++q) {
     const int n = t + TPL * q;
-    if (va) a.p[la * N + n] = z[q].x;
-    if (vb) a.p[lb * N + n] = z[q].y;
+    if (BS && n >= nl) continue;
+    if (va) a.p[la * nl + n] = z[q].x;
+    if (vb) a.p[lb * nl + n] = z[q].y;
   }
 }
 
-template <typename T, int N>
+template <typename T, int N, bool BS = false>
 __global__ void __launch_bounds__(256) k_c2r_x(C2RArgs<T> a) {
-  c2r_body<T, N>(a, blockIdx.x);
+  c2r_body<T, N, BS>(a, blockIdx.x);
 }
 
 template <typename T, int N>

@@ -12,15 +12,28 @@
   X(80) X(81) X(96) X(100) X(125) X(128) X(135) X(144) X(160) X(192) X(200) X(225) X(243) X(256) X(320)  \
   X(384) X(400) X(512) X(640) X(768) X(1024) X(2048) X(4096)
 
+// Power-of-two plans that carry lines of any other length (Bluestein).
+#define KW_BS_SIZES(X) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096)
+
 namespace kw {
 
 bool size_supported(long long n);
+// Plan length of a line of length n without a compiled plan: the power of two
+// n_plan >= 2n - 1 (at least 8) on which its chirp-z transform runs; 0 if n is
+// longer than the largest plan allows (n > 2048).
+inline long long bluestein_plan(long long n) {
+  if (n < 1) return 0;
+  long long m = 8;
+  while (m < 2 * n - 1) m <<= 1;
+  return m <= 4096 ? m : 0;
+}
 // grid of the adjoint's reduction kernels (one partial-sum slot per block)
 inline unsigned adj_reduce_blocks(long long n) {
   const long long nb = (n + 255) / 256;
   return nb < 1 ? 1u : (nb < 1024 ? (unsigned)nb : 1024u);
 }
-int plan_tpl(int n, int variant = 0);  // variant 1: contiguous x-line passes, 2: complex-to-real x pass
+// variant 1: contiguous x-line passes, 2: complex-to-real x pass, 4: both for chirp-z lines
+int plan_tpl(int n, int variant = 0);
 int plan_pitch(int n);
 
 // Return hipSuccess or the launch error; unsupported lengths give hipErrorInvalidValue.
@@ -30,6 +43,13 @@ template <typename T>
 hipError_t launch_spec(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<T>& a);
 template <typename T>
 hipError_t launch_c2r(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const C2RArgs<T>& a);
+// Bluestein forms (n_plan from bluestein_plan; the args carry the line length and chirp tables).
+template <typename T>
+hipError_t launch_dir_bs(int n_plan, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const PassArgs<T>& a);
+template <typename T>
+hipError_t launch_spec_bs(int n_plan, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<T>& a);
+template <typename T>
+hipError_t launch_c2r_bs(int n_plan, dim3 g, dim3 b, size_t sh, hipStream_t s, const C2RArgs<T>& a);
 template <typename T>
 hipError_t launch_record(unsigned nb, hipStream_t s, const RecArgs<T>& a);
 template <typename T>

@@ -18,7 +18,8 @@ int plan_tpl(int n, int variant) {
   switch (n) {
 #define X(n_) \
   case n_:    \
-    return variant == 2 ? Plan<n_, 2>::TPL : (variant ? Plan<n_, 1>::TPL : Plan<n_, 0>::TPL);
+    return variant == 4 ? Plan<n_, 4>::TPL                                                  \
+                        : (variant == 2 ? Plan<n_, 2>::TPL : (variant ? Plan<n_, 1>::TPL : Plan<n_, 0>::TPL));
     KW_SIZES(X)
 #undef X
   }

@@ -33,7 +33,9 @@ def test_library_exports_every_declared_symbol():
     assert lib.kw_abi_version() == 1
     for n in (8, 9, 15, 24, 25, 27, 45, 81, 100, 125, 128, 135, 225, 243, 256, 384, 400, 512, 768, 1024, 4096):
         assert lib.kw_fft_length_supported(n) == 1
-    for n in (7, 11, 101, 1000, 8192):
+    for n in (2, 7, 11, 101, 257, 1000, 2048 - 1):
+        assert lib.kw_fft_length_supported(n) == 2     # chirp-z lines on a power-of-two plan
+    for n in (0, 2049, 3001, 8192):
         assert lib.kw_fft_length_supported(n) == 0
 
 
@@ -52,7 +54,8 @@ def test_grid_lane_offset_limit():
     assert ok((1024, 1024, 512), 8) == 0           # 528*1024*511 complex128 = 4.4e9 B
     assert ok((2048, 2048, 256), 4) == 0
     assert ok((4096, 4096), 8) == 1                # 2-D lines stay far below the limit
-    assert ok((1000, 1000), 4) == 0                # no line-FFT plan
+    assert ok((1000, 1000), 4) == 1                # chirp-z lines
+    assert ok((3001, 64), 4) == 0                  # longer than the chirp-z limit without a plan
     assert ok((256, 256, 256), 2) == 0             # unknown precision
 
 
