@@ -433,11 +433,47 @@ __device__ __forceinline__ void lds_sync() {
 }
 
 template <typename T, int N, int V, int TPL, bool INV, int Ns, int OFF, bool WAVE, int R, int... Rest>
+struct StockhamPass;
+// the pass after one (void after the last)
+template <typename T, int N, int V, int TPL, bool INV, int Ns, int OFF, bool WAVE, int... Rs>
+struct StockhamNext {
+  using type = StockhamPass<T, N, V, TPL, INV, Ns, OFF, WAVE, Rs...>;
+};
+template <typename T, int N, int V, int TPL, bool INV, int Ns, int OFF, bool WAVE>
+struct StockhamNext<T, N, V, TPL, INV, Ns, OFF, WAVE> {
+  using type = void;
+};
+
+template <typename T, int N, int V, int TPL, bool INV, int Ns, int OFF, bool WAVE, int R, int... Rest>
 struct StockhamPass {
   static constexpr int E = N / TPL;
   static constexpr int NB = E / R;
   static_assert(E % R == 0, "radix must divide the per-thread element count");
+  // this pass reads its twiddles from the table (plans too large to keep
+  // them in registers); NW of them per thread
+  static constexpr bool TAB = !Twiddles<T, N, V>::REG && Ns > 1;
+  static constexpr int NW = TAB ? NB * (R - 1) : 1;
+  using Next =
+      typename StockhamNext<T, N, V, TPL, INV, Ns * R, (Ns > 1 ? OFF + NB * (R - 1) : OFF), WAVE, Rest...>::type;
+
+  __device__ __forceinline__ static void fetch(Cx<T> (&w)[NW], const Twiddles<T, N, V>& tw, int t) {
+    if constexpr (TAB) {
+#pragma unroll
+      for (int m = 0; m < NB; ++m) {
+        const int jm = (t + TPL * m) % Ns;
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[m * (R - 1) + (r - 1)] = tw.g[jm * r * (N / (Ns * R))];
+      }
+    }
+  }
   __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Twiddles<T, N, V>& tw, int t) {
+    Cx<T> w[NW];
+    fetch(w, tw, t);
+    body(v, lds, tw, t, w);
+  }
+  // w: this pass's table twiddles, fetched by the caller
+  __device__ __forceinline__ static void body(Cx<T> (&v)[E], Cx<T>* lds, const Twiddles<T, N, V>& tw, int t,
+                                              const Cx<T> (&w)[NW]) {
     constexpr bool LAST = sizeof...(Rest) == 0;
     static_assert(!LAST || Ns * R == N, "radix product must equal N");
 #pragma unroll
@@ -447,15 +483,14 @@ struct StockhamPass {
       for (int r = 0; r < R; ++r) x[r] = v[m + r * NB];
       const int j = t + TPL * m;
       if constexpr (Ns > 1) {
-        const int jm = j % Ns;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          Cx<T> w;
+          Cx<T> wr;
           if constexpr (Twiddles<T, N, V>::REG)
-            w = tw.w[OFF + m * (R - 1) + (r - 1)];
+            wr = tw.w[OFF + m * (R - 1) + (r - 1)];
           else
-            w = tw.g[jm * r * (N / (Ns * R))];
-          x[r] = INV ? cmulc(x[r], w) : cmul(x[r], w);
+            wr = w[m * (R - 1) + (r - 1)];
+          x[r] = INV ? cmulc(x[r], wr) : cmul(x[r], wr);
         }
       }
       DFT<T, R, INV>::run(x);
@@ -469,12 +504,15 @@ struct StockhamPass {
       }
     }
     if constexpr (!LAST) {
+      // the next pass's table twiddles are requested before the exchange so
+      // their latency overlaps the barrier and the LDS reads
+      Cx<T> wn[Next::NW];
+      Next::fetch(wn, tw, t);
       lds_sync<WAVE>();
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = lds[LdsLayout<N>::at(t + TPL * q)];
       lds_sync<WAVE>();
-      StockhamPass<T, N, V, TPL, INV, Ns * R, (Ns > 1 ? OFF + NB * (R - 1) : OFF), WAVE, Rest...>::run(v, lds, tw,
-                                                                                                        t);
+      Next::body(v, lds, tw, t, wn);
     }
   }
 };
@@ -486,7 +524,13 @@ struct LineFFTImpl<T, N, V, INV, WAVE, Radices<Rs...>> {
   static constexpr int TPL = Plan<N, V>::TPL;
   static constexpr int E = Plan<N, V>::E;
   __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Twiddles<T, N, V>& tw, int t) {
-    StockhamPass<T, N, V, TPL, INV, 1, 0, WAVE, Rs...>::run(v, lds, tw, t);
+    // An opaque copy of the thread index: the address arithmetic of this
+    // transform is then not merged with that of the kernel's other
+    // transforms, which would keep every LDS and twiddle address live
+    // across all of them (registers, hence occupancy).
+    int tt = t;
+    asm volatile("" : "+v"(tt));
+    StockhamPass<T, N, V, TPL, INV, 1, 0, WAVE, Rs...>::run(v, lds, tw, tt);
   }
 };
 

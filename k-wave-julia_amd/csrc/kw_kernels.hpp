@@ -142,6 +142,24 @@ __device__ __forceinline__ U& at32(U* base, unsigned i) {
   return *reinterpret_cast<U*>(reinterpret_cast<char*>(base) + (unsigned)(i * (unsigned)sizeof(U)));
 }
 
+// Argument records and workgroup maps of the batched kernels are written by
+// the host before a launch and only read by kernels.  Read through the
+// constant address space they become scalar loads, and the pointers they
+// hold are then taken to be global (as for kernel arguments), so field
+// accesses are global scalar-base + vector-offset instructions instead of
+// flat ones (64-bit per-lane addresses, LDS wait counter shared with them).
+// (The host pass of the compiler only checks kernel bodies; it keeps plain
+// pointers.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define KW_CONST __attribute__((address_space(4)))
+#else
+#define KW_CONST
+#endif
+template <typename U>
+__device__ __forceinline__ const KW_CONST U* cmem(const U* p) {
+  return (const KW_CONST U*)p;
+}
+
 // Addresses are a per-workgroup 64-bit base (uniform) plus 32-bit lane
 // offsets, so every load is a scalar-base + vector-offset access.
 // BS: the line has a runtime length nl < N (Bluestein lines on the N-point
@@ -627,8 +645,9 @@ __global__ void __launch_bounds__(512) k_dir_pass_batch(const PassArgs<T>* __res
                                                         const int* __restrict__ wg_sim,
                                                         const int* __restrict__ wg_first, const int* t_dev) {
   const int b = CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x);
-  const int s = wg_sim[b];
-  dir_pass_body<T, N, CONTIG, TWO>(args[s], b - wg_first[s], *t_dev);
+  const int s = cmem(wg_sim)[b];
+  const PassArgs<T> a = cmem(args)[s];
+  dir_pass_body<T, N, CONTIG, TWO>(a, b - cmem(wg_first)[s], *cmem(t_dev));
 }
 
 // ---------------------------------------------------------------------------
@@ -793,8 +812,9 @@ __global__ void __launch_bounds__(512) k_spec_pass_batch(const SpecArgs<T>* __re
                                                          const int* __restrict__ wg_sim,
                                                          const int* __restrict__ wg_first) {
   const int b = strided_wg<T, N>(blockIdx.x);
-  const int s = wg_sim[b];
-  spec_pass_body<T, N, MODE>(args[s], b - wg_first[s]);
+  const int s = cmem(wg_sim)[b];
+  const SpecArgs<T> a = cmem(args)[s];
+  spec_pass_body<T, N, MODE>(a, b - cmem(wg_first)[s]);
 }
 
 // ---------------------------------------------------------------------------
@@ -884,8 +904,9 @@ template <typename T, int N>
 __global__ void __launch_bounds__(256) k_c2r_x_batch(const C2RArgs<T>* __restrict__ args,
                                                      const int* __restrict__ wg_sim,
                                                      const int* __restrict__ wg_first) {
-  const int s = wg_sim[blockIdx.x];
-  c2r_body<T, N>(args[s], blockIdx.x - wg_first[s]);
+  const int s = cmem(wg_sim)[blockIdx.x];
+  const C2RArgs<T> a = cmem(args)[s];
+  c2r_body<T, N>(a, blockIdx.x - cmem(wg_first)[s]);
 }
 
 // ---------------------------------------------------------------------------
@@ -982,8 +1003,9 @@ __global__ void __launch_bounds__(256) k_record_batch(const RecArgs<T>* __restri
                                                       const int* t_dev) {
   const int J = blockIdx.x * blockDim.x + threadIdx.x;
   if (J >= n_total) return;
-  const int s = sens_sim[J];
-  record_body<T>(args[s], J - sens_first[s], *t_dev);
+  const int s = cmem(sens_sim)[J];
+  const RecArgs<T> a = cmem(args)[s];
+  record_body<T>(a, J - cmem(sens_first)[s], *cmem(t_dev));
 }
 
 // Advance the device-side step counter read by the batched kernels (one
