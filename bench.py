@@ -133,7 +133,8 @@ def cpu_baseline(n_steps_max: int = 20, budget_s: float = 20.0):
 
 def bench_batch(a):
     """64 uneven 2-D firings on 256^2..1024^2 grids; batched (one launch per pass per line
-    length, hipGraph step) against the same simulations run one after another."""
+    length, line lengths on concurrent streams, hipGraph step) against the same simulations
+    run one after another."""
     import numpy as np
     import kwave_amd as K
     from kwave_amd.configs import uneven_2d_batch

@@ -1395,6 +1395,7 @@ class Batch final : public BatchBase {
     std::vector<int> t0(1, 0);
     upload(t_dev_, t0);
     build_groups();
+    build_lanes();
     build_record();
   }
   ~Batch() override {
@@ -1402,6 +1403,11 @@ class Batch final : public BatchBase {
     if (stream_) (void)hipStreamSynchronize(stream_);
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
+    for (hipEvent_t e : done_)
+      if (e) (void)hipEventDestroy(e);
+    if (tick_ev_) (void)hipEventDestroy(tick_ev_);
+    for (hipStream_t l : lanes_)
+      if (l) (void)hipStreamDestroy(l);
     if (stream_) (void)hipStreamDestroy(stream_);
   }
   void run(long long n) override {
@@ -1488,6 +1494,9 @@ class Batch final : public BatchBase {
     unsigned nwg = 0;
     bool two = false;
     DevBuf args, wg_sim, wg_first;
+    std::vector<int> sims;   // simulations with this line length for this pass
+    int lane = 0;            // stream lane of the concurrent step
+    std::vector<int> deps;   // earlier groups on other lanes it waits for
   };
 
   // one group per (pass kind, line length), in step order
@@ -1522,6 +1531,7 @@ class Batch final : public BatchBase {
           g.block = c.block;
           g.shmem = c.shmem;
           g.req = c.req;
+          g.sims.push_back((int)i);
           wg_first[i] = (int)g.nwg;
           for (unsigned w = 0; w < c.grid.x; ++w) wg_sim.push_back((int)i);
           g.nwg += c.grid.x;
@@ -1602,33 +1612,101 @@ class Batch final : public BatchBase {
     upload(sens_first_, sens_first);
   }
 
+  // Groups touching disjoint simulations are independent, so one step is a
+  // small DAG: each group depends on the latest earlier group that shares a
+  // simulation with it.  Chains of groups run on up to kLanes streams (the
+  // device's default count of hardware queues), the first groups of the chains
+  // balanced by workgroup count, so small line lengths run beside large ones
+  // instead of after them.  Lane 0 is the batch stream: it runs the step
+  // counter first and the sensor gather last, after joining every lane.
+  static constexpr int kLanes = 4;
+  void build_lanes() {
+    const int ng = (int)groups_.size();
+    std::vector<int> last(sims_.size(), -1);
+    std::vector<long long> load(kLanes, 0);
+    std::vector<int> lane_last(kLanes, -1);
+    int used = 1;
+    for (int k = 0; k < ng; ++k) {
+      Group& g = *groups_[k];
+      std::vector<int> deps;
+      for (int i : g.sims)
+        if (last[i] >= 0 && std::find(deps.begin(), deps.end(), last[i]) == deps.end()) deps.push_back(last[i]);
+      for (int i : g.sims) last[i] = k;
+      int lane = -1;
+      // continue a dependency's chain when that lane's latest group is the dependency
+      for (int d : deps)
+        if (lane_last[groups_[d]->lane] == d && (lane < 0 || load[groups_[d]->lane] > load[lane])) lane = groups_[d]->lane;
+      if (lane < 0) {   // a new chain: the least loaded lane
+        lane = 0;
+        for (int l = 1; l < kLanes; ++l)
+          if (load[l] < load[lane]) lane = l;
+      }
+      g.lane = lane;
+      for (int d : deps)
+        if (groups_[d]->lane != lane) g.deps.push_back(d);
+      load[lane] += g.nwg;
+      lane_last[lane] = k;
+      used = std::max(used, lane + 1);
+    }
+    lanes_.assign(used, nullptr);
+    lanes_[0] = stream_;
+    for (int l = 1; l < used; ++l) HIPC(hipStreamCreate(&lanes_[l]));
+    done_.assign(ng, nullptr);
+    for (int k = 0; k < ng; ++k) HIPC(hipEventCreateWithFlags(&done_[k], hipEventDisableTiming));
+    HIPC(hipEventCreateWithFlags(&tick_ev_, hipEventDisableTiming));
+  }
+
+  void launch_group(const Group& g, hipStream_t st) {
+    switch (g.kind) {
+      case DIR_Y: case DIR_Z: case DIR_X:
+        HIPC(launch_dir_batch<T>(g.n, g.kind == DIR_X, g.kind == DIR_X && g.two, g.nwg, g.block, g.shmem, st, reinterpret_cast<PassArgs<T>*>(g.args.p),
+                                 reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p), reinterpret_cast<int*>(t_dev_.p)));
+        break;
+      case C2R:
+        HIPC(launch_c2r_batch<T>(g.n, g.nwg, g.block, g.shmem, st, reinterpret_cast<C2RArgs<T>*>(g.args.p), reinterpret_cast<int*>(g.wg_sim.p),
+                                 reinterpret_cast<int*>(g.wg_first.p)));
+        break;
+      default: {
+        const int mode = g.kind == SPEC_Y_FWD ? 0 : (g.kind == SPEC_Y_INV ? 1 : 2);
+        HIPC(launch_spec_batch<T>(g.n, mode, g.two && mode == 2, g.nwg, g.block, g.shmem, st,
+                                  reinterpret_cast<SpecArgs<T>*>(g.args.p), reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p)));
+      }
+    }
+  }
+
+  // One step.  With `ev` (autotuner trials) every group runs on the batch
+  // stream between its own timing events; otherwise on its lane.
   void enqueue_step(std::vector<std::pair<hipEvent_t, hipEvent_t>>* ev = nullptr) {
     HIPC(launch_tick(stream_, reinterpret_cast<int*>(t_dev_.p)));
-    for (auto& gp : groups_) {
-      Group& g = *gp;
+    const bool lanes = ev == nullptr && lanes_.size() > 1;
+    if (lanes) HIPC(hipEventRecord(tick_ev_, stream_));
+    std::vector<int> lane_last(lanes_.size(), -1);
+    for (size_t k = 0; k < groups_.size(); ++k) {
+      Group& g = *groups_[k];
       if (ev) {
         ev->emplace_back(nullptr, nullptr);
         HIPC(hipEventCreate(&ev->back().first));
         HIPC(hipEventCreate(&ev->back().second));
         HIPC(hipEventRecord(ev->back().first, stream_));
+        launch_group(g, stream_);
+        HIPC(hipEventRecord(ev->back().second, stream_));
+        continue;
       }
-      switch (g.kind) {
-        case DIR_Y: case DIR_Z: case DIR_X:
-          HIPC(launch_dir_batch<T>(g.n, g.kind == DIR_X, g.kind == DIR_X && g.two, g.nwg, g.block, g.shmem, stream_, reinterpret_cast<PassArgs<T>*>(g.args.p),
-                                   reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p), reinterpret_cast<int*>(t_dev_.p)));
-          break;
-        case C2R:
-          HIPC(launch_c2r_batch<T>(g.n, g.nwg, g.block, g.shmem, stream_, reinterpret_cast<C2RArgs<T>*>(g.args.p), reinterpret_cast<int*>(g.wg_sim.p),
-                                   reinterpret_cast<int*>(g.wg_first.p)));
-          break;
-        default: {
-          const int mode = g.kind == SPEC_Y_FWD ? 0 : (g.kind == SPEC_Y_INV ? 1 : 2);
-          HIPC(launch_spec_batch<T>(g.n, mode, g.two && mode == 2, g.nwg, g.block, g.shmem, stream_,
-                                    reinterpret_cast<SpecArgs<T>*>(g.args.p), reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p)));
-        }
+      hipStream_t st = lanes ? lanes_[g.lane] : stream_;
+      if (lanes) {
+        // a lane joins the step after the counter, before its first group
+        if (g.lane > 0 && lane_last[g.lane] < 0) HIPC(hipStreamWaitEvent(st, tick_ev_, 0));
+        for (int d : g.deps) HIPC(hipStreamWaitEvent(st, done_[d], 0));
       }
-      if (ev) HIPC(hipEventRecord(ev->back().second, stream_));
+      launch_group(g, st);
+      if (lanes) {
+        HIPC(hipEventRecord(done_[k], st));
+        lane_last[g.lane] = (int)k;
+      }
     }
+    if (lanes)   // join: the gather and the next step follow every lane
+      for (size_t l = 1; l < lanes_.size(); ++l)
+        if (lane_last[l] >= 0) HIPC(hipStreamWaitEvent(stream_, done_[lane_last[l]], 0));
     if (n_sens_ > 0)
       HIPC(launch_record_batch<T>(stream_, reinterpret_cast<RecArgs<T>*>(rec_args_.p), reinterpret_cast<int*>(sens_sim_.p), reinterpret_cast<int*>(sens_first_.p),
                                   n_sens_, reinterpret_cast<int*>(t_dev_.p)));
@@ -1653,6 +1731,9 @@ class Batch final : public BatchBase {
   hipStream_t stream_ = nullptr;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
+  std::vector<hipStream_t> lanes_;   // lanes_[0] == stream_
+  std::vector<hipEvent_t> done_;     // per group: recorded after its launch
+  hipEvent_t tick_ev_ = nullptr;
   std::vector<std::unique_ptr<Sim<T>>> sims_;
   std::vector<std::unique_ptr<Group>> groups_;
   DevBuf t_dev_, rec_args_, sens_sim_, sens_first_;

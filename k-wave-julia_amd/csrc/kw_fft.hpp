@@ -517,19 +517,20 @@ struct StockhamPass {
   }
 };
 
-template <typename T, int N, int V, bool INV, bool WAVE, typename RL>
+template <typename T, int N, int V, bool INV, bool WAVE, bool OPQ, typename RL>
 struct LineFFTImpl;
-template <typename T, int N, int V, bool INV, bool WAVE, int... Rs>
-struct LineFFTImpl<T, N, V, INV, WAVE, Radices<Rs...>> {
+template <typename T, int N, int V, bool INV, bool WAVE, bool OPQ, int... Rs>
+struct LineFFTImpl<T, N, V, INV, WAVE, OPQ, Radices<Rs...>> {
   static constexpr int TPL = Plan<N, V>::TPL;
   static constexpr int E = Plan<N, V>::E;
   __device__ __forceinline__ static void run(Cx<T> (&v)[E], Cx<T>* lds, const Twiddles<T, N, V>& tw, int t) {
-    // An opaque copy of the thread index: the address arithmetic of this
-    // transform is then not merged with that of the kernel's other
-    // transforms, which would keep every LDS and twiddle address live
-    // across all of them (registers, hence occupancy).
+    // OPQ: an opaque copy of the thread index, so the address arithmetic of
+    // this transform is not merged with that of the kernel's other
+    // transforms (which keeps every LDS and twiddle address live across all
+    // of them: fewer registers and more waves per SIMD for some kernels, more
+    // address arithmetic for all).
     int tt = t;
-    asm volatile("" : "+v"(tt));
+    if constexpr (OPQ) asm volatile("" : "+v"(tt));
     StockhamPass<T, N, V, TPL, INV, 1, 0, WAVE, Rs...>::run(v, lds, tw, tt);
   }
 };
@@ -546,11 +547,11 @@ constexpr bool wave_local_lines() {
 // Every thread of the workgroup must call it (it contains barriers).  WAVE:
 // the caller's lines are wave-local (consecutive threads, TPL dividing 64),
 // so the exchanges synchronise the wavefront only.
-template <typename T, int N, bool INV, int V = 0, bool WAVE = false>
+template <typename T, int N, bool INV, int V = 0, bool WAVE = false, bool OPQ = false>
 __device__ __forceinline__ void line_fft(Cx<T> (&v)[(Plan<N, V>::E)], Cx<T>* lds, const Twiddles<T, N, V>& tw,
                                          int t) {
   static_assert(!WAVE || 64 % Plan<N, V>::TPL == 0, "wave-local lines need TPL dividing 64");
-  LineFFTImpl<T, N, V, INV, WAVE, typename Plan<N, V>::R>::run(v, lds, tw, t);
+  LineFFTImpl<T, N, V, INV, WAVE, OPQ, typename Plan<N, V>::R>::run(v, lds, tw, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -573,11 +574,11 @@ struct Chirp {
 // Forward (INV=false) or unnormalised inverse DFT of a line of length ch.n held
 // in v (stride-TPL layout of the N-point plan; slots j >= n hold zero on exit
 // and are ignored on entry).  BS=false is the plain N-point line_fft.
-template <typename T, int N, bool INV, int V = 0, bool WAVE = false, bool BS = false>
+template <typename T, int N, bool INV, int V = 0, bool WAVE = false, bool BS = false, bool OPQ = false>
 __device__ __forceinline__ void line_dft(Cx<T> (&v)[(Plan<N, V>::E)], Cx<T>* lds, const Twiddles<T, N, V>& tw, int t,
                                          const Chirp<T>& ch) {
   if constexpr (!BS) {
-    line_fft<T, N, INV, V, WAVE>(v, lds, tw, t);
+    line_fft<T, N, INV, V, WAVE, OPQ>(v, lds, tw, t);
   } else {
     constexpr int TPL = Plan<N, V>::TPL, E = Plan<N, V>::E;
     const int n = ch.n;
@@ -586,11 +587,11 @@ __device__ __forceinline__ void line_dft(Cx<T> (&v)[(Plan<N, V>::E)], Cx<T>* lds
       const int j = t + TPL * q;
       v[q] = j < n ? (INV ? cmulc(v[q], ch.c[j]) : cmul(v[q], ch.c[j])) : Cx<T>{0, 0};
     }
-    line_fft<T, N, false, V, WAVE>(v, lds, tw, t);
+    line_fft<T, N, false, V, WAVE, OPQ>(v, lds, tw, t);
     const Cx<T>* g = INV ? ch.gb : ch.gf;
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cmul(v[q], g[t + TPL * q]);
-    line_fft<T, N, true, V, WAVE>(v, lds, tw, t);
+    line_fft<T, N, true, V, WAVE, OPQ>(v, lds, tw, t);
 #pragma unroll
     for (int q = 0; q < E; ++q) {
       const int j = t + TPL * q;

@@ -393,10 +393,22 @@ __device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)
 // TWO (x pass only): the variant that can also form the second spectrum of
 // the homogeneous :stokes branch; without it the kernel carries no registers
 // for that path.
+// Whether the directional pass keeps each transform's address arithmetic to
+// itself (OPQ in line_fft).  Compiled for gfx950, that never costs the x pass
+// occupancy and raises it (or removes spills) for most lengths; strided
+// passes on 16-element plans (128 ... 4096 points) run at two waves per SIMD
+// either way, and there the merged addresses are faster (512-point strided
+// pass of the 512^3 step: 0.94 ms merged, 1.00 ms separate).
+template <typename T, int N, bool CONTIG>
+constexpr bool dir_opaque() {
+  return CONTIG || Plan<N, 0>::E != 16;
+}
+
 template <typename T, int N, bool CONTIG, bool TWO = false, bool BS = false>
 __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int t_index) {
   constexpr int V = CONTIG ? (BS ? 4 : 1) : 0;  // plan variant of contiguous x-line passes
   constexpr bool WL = CONTIG && wave_local_lines<N, V>();   // lines inside one wavefront
+  constexpr bool OPQ = dir_opaque<T, N, CONTIG>();
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -409,10 +421,10 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
   // --- grad p (staggered +) : rfft -> i k e^{+ikh/2} -> irfft
   Cx<T> v[E];
   L.load(a.p, v);
-  line_dft<T, N, false, V, WL, BS>(v, lds, tw, L.t, a.ch);
+  line_dft<T, N, false, V, WL, BS, OPQ>(v, lds, tw, L.t, a.ch);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], L.tab(a.Dp, q));
-  line_dft<T, N, true, V, WL, BS>(v, lds, tw, L.t, a.ch);
+  line_dft<T, N, true, V, WL, BS, OPQ>(v, lds, tw, L.t, a.ch);
 
   // --- velocity update  u = pml_sg (pml_sg u - dt/rho0_sg dp)
   Cx<T> r0[E];
@@ -467,10 +479,10 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
   L.store(a.u, v);
 
   // --- div u (staggered -)
-  line_dft<T, N, false, V, WL, BS>(v, lds, tw, L.t, a.ch);
+  line_dft<T, N, false, V, WL, BS, OPQ>(v, lds, tw, L.t, a.ch);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], L.tab(a.Dm, q));
-  line_dft<T, N, true, V, WL, BS>(v, lds, tw, L.t, a.ch);
+  line_dft<T, N, true, V, WL, BS, OPQ>(v, lds, tw, L.t, a.ch);
   if constexpr (!CONTIG) {
     // axisymmetric radial pass: du_r/dr + u_r / r, u_r as stored above (an L2 hit)
     if (a.inv_r != nullptr) {
@@ -608,7 +620,7 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
       Cx<T> z[E];
 #pragma unroll
       for (int q = 0; q < E; ++q) z[q] = s == 0 ? f[q] : g[TWO ? q : 0];
-      line_dft<T, N, false, V, WL, BS>(z, lds, tw, L.t, a.ch);
+      line_dft<T, N, false, V, WL, BS, OPQ>(z, lds, tw, L.t, a.ch);
 #pragma unroll
       for (int q = 0; q < E; ++q) lds[LdsLayout<N>::at(L.n(q))] = z[q];
       lds_sync<WL>();
@@ -685,6 +697,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
   constexpr int TPL = Plan<N>::TPL;
   constexpr int E = Plan<N>::E;
   constexpr int P = LdsLayout<N>::PITCH;
+  constexpr bool OPQ = MODE != 1;   // every mode but the plain inverse has two transforms
   extern __shared__ __align__(16) unsigned char smem_raw[];
   Cx<T>* lds_all = reinterpret_cast<Cx<T>*>(smem_raw);
   const int lpw = blockDim.x / TPL;
@@ -706,14 +719,14 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = (valid && ok(q)) ? at32(S1, off(q)) : Cx<T>{0, 0};
   if constexpr (MODE == 1) {
-    line_dft<T, N, true, 0, false, BS>(v, lds, tw, t, a.ch);
+    line_dft<T, N, true, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
   } else if constexpr (MODE == 3) {
-    line_dft<T, N, false, 0, false, BS>(v, lds, tw, t, a.ch);
+    line_dft<T, N, false, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cscale(v[q], ok(q) ? a.W[t + TPL * q] : (T)0);
-    line_dft<T, N, true, 0, false, BS>(v, lds, tw, t, a.ch);
+    line_dft<T, N, true, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
   } else if constexpr (MODE == 0) {
-    line_dft<T, N, false, 0, false, BS>(v, lds, tw, t, a.ch);
+    line_dft<T, N, false, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
     if (a.S2) {
       Cx<T>* __restrict__ S2 = a.S2 + wgb;
       if (valid) {
@@ -723,7 +736,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
       }
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = (valid && ok(q)) ? at32(S2, off(q)) : Cx<T>{0, 0};
-      line_dft<T, N, false, 0, false, BS>(v, lds, tw, t, a.ch);
+      line_dft<T, N, false, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
       if (valid) {
 #pragma unroll
         for (int q = 0; q < E; ++q)
@@ -732,7 +745,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
       return;
     }
   } else if constexpr (MODE == 5) {
-    line_dft<T, N, false, 0, false, BS>(v, lds, tw, t, a.ch);
+    line_dft<T, N, false, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
     int io = o;
     if (a.axis == 1)
       io = a.Nz > 1 ? (io > a.Nz - io ? a.Nz - io : io) : 0;   // 2-D: outer index = stacked firing
@@ -749,16 +762,16 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
       w[q] = cscale(v[q], m2);
       v[q] = cscale(v[q], m1);
     }
-    line_dft<T, N, true, 0, false, BS>(w, lds, tw, t, a.ch);
+    line_dft<T, N, true, 0, false, BS, OPQ>(w, lds, tw, t, a.ch);
     if (valid) {
       Cx<T>* __restrict__ S2 = a.S2 + wgb;
 #pragma unroll
       for (int q = 0; q < E; ++q)
         if (ok(q)) at32(S2, off(q)) = w[q];
     }
-    line_dft<T, N, true, 0, false, BS>(v, lds, tw, t, a.ch);
+    line_dft<T, N, true, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
   } else {
-    line_dft<T, N, false, 0, false, BS>(v, lds, tw, t, a.ch);
+    line_dft<T, N, false, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
     Cx<T> w[MODE == 4 ? E : 1];
     // In a batch, a MODE 4 launch also covers simulations without a second
     // spectrum (S2 and M2 null): those take w = 0 and never touch S2/M2.
@@ -772,7 +785,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
 #pragma unroll
         for (int q = 0; q < E; ++q) w[q] = Cx<T>{0, 0};
       }
-      line_dft<T, N, false, 0, false, BS>(w, lds, tw, t, a.ch);
+      line_dft<T, N, false, 0, false, BS, OPQ>(w, lds, tw, t, a.ch);
     }
     // octant index |i| of the outer coordinate (z for y lines, y for z lines)
     int io = o;
@@ -793,7 +806,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
       }
       v[q] = r;
     }
-    line_dft<T, N, true, 0, false, BS>(v, lds, tw, t, a.ch);
+    line_dft<T, N, true, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
   }
   if (valid) {
 #pragma unroll
