@@ -1668,7 +1668,7 @@ class Batch final : public BatchBase {
         break;
       default: {
         const int mode = g.kind == SPEC_Y_FWD ? 0 : (g.kind == SPEC_Y_INV ? 1 : 2);
-        HIPC(launch_spec_batch<T>(g.n, mode, g.two && mode == 2, g.nwg, g.block, g.shmem, st,
+        HIPC(launch_spec_batch<T>(g.n, mode, g.two && mode != 1, g.nwg, g.block, g.shmem, st,
                                   reinterpret_cast<SpecArgs<T>*>(g.args.p), reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p)));
       }
     }

@@ -688,7 +688,7 @@ struct SpecArgs {
   Chirp<T> ch;
 };
 
-// MODE: 0 forward (both spectra when S2 is set), 1 inverse, 2 fwd*M1*inv,
+// MODE: 0 forward, 6 forward of both spectra where S2 is set, 1 inverse, 2 fwd*M1*inv,
 // 4 fwd*(M1 S1 + M2 S2)*inv (:stokes), 3 fwd*W*inv, 5 fwd then S1 = inv(M1 .),
 // S2 = inv(M2 .) (the adjoint's K and dK/dc_ref applied to one spectrum).  Specialising on the mode
 // keeps the plain transforms at a small register footprint.
@@ -725,9 +725,12 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cscale(v[q], ok(q) ? a.W[t + TPL * q] : (T)0);
     line_dft<T, N, true, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
-  } else if constexpr (MODE == 0) {
+  } else if constexpr (MODE == 0 || MODE == 6) {
+    // 6: forward of S1 and, where the simulation has one, of S2 (the
+    // second-spectrum code would otherwise cost plain forward passes
+    // registers, hence occupancy)
     line_dft<T, N, false, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
-    if (a.S2) {
+    if (MODE == 6 && a.S2) {
       Cx<T>* __restrict__ S2 = a.S2 + wgb;
       if (valid) {
 #pragma unroll

@@ -31,7 +31,10 @@ static hipError_t spec_batch_mode(int mode, bool two, unsigned nwg, dim3 b, size
                                   const SpecArgs<double>* args, const int* wg_sim, const int* wg_first) {
   switch (mode) {
     case 0:
-      hipLaunchKernelGGL((k_spec_pass_batch<double, N, 0>), dim3(nwg), b, sh, s, args, wg_sim, wg_first);
+      if (two)
+        hipLaunchKernelGGL((k_spec_pass_batch<double, N, 6>), dim3(nwg), b, sh, s, args, wg_sim, wg_first);
+      else
+        hipLaunchKernelGGL((k_spec_pass_batch<double, N, 0>), dim3(nwg), b, sh, s, args, wg_sim, wg_first);
       break;
     case 1:
       hipLaunchKernelGGL((k_spec_pass_batch<double, N, 1>), dim3(nwg), b, sh, s, args, wg_sim, wg_first);

@@ -27,7 +27,10 @@ template <int N>
 static hipError_t spec_mode_bs(int mode, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<float>& a) {
   switch (mode) {
     case 0:
-      hipLaunchKernelGGL((k_spec_pass<float, N, 0, true>), g, b, sh, s, a);
+      if (a.S2)
+        hipLaunchKernelGGL((k_spec_pass<float, N, 6, true>), g, b, sh, s, a);
+      else
+        hipLaunchKernelGGL((k_spec_pass<float, N, 0, true>), g, b, sh, s, a);
       break;
     case 1:
       hipLaunchKernelGGL((k_spec_pass<float, N, 1, true>), g, b, sh, s, a);

@@ -27,7 +27,10 @@ template <int N>
 static hipError_t spec_mode_bs(int mode, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<double>& a) {
   switch (mode) {
     case 0:
-      hipLaunchKernelGGL((k_spec_pass<double, N, 0, true>), g, b, sh, s, a);
+      if (a.S2)
+        hipLaunchKernelGGL((k_spec_pass<double, N, 6, true>), g, b, sh, s, a);
+      else
+        hipLaunchKernelGGL((k_spec_pass<double, N, 0, true>), g, b, sh, s, a);
       break;
     case 1:
       hipLaunchKernelGGL((k_spec_pass<double, N, 1, true>), g, b, sh, s, a);

@@ -7,7 +7,10 @@ template <int N, typename T>
 static hipError_t spec_mode(int mode, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<T>& a) {
   switch (mode) {
     case 0:
-      hipLaunchKernelGGL((k_spec_pass<T, N, 0>), g, b, sh, s, a);
+      if (a.S2)
+        hipLaunchKernelGGL((k_spec_pass<T, N, 6>), g, b, sh, s, a);
+      else
+        hipLaunchKernelGGL((k_spec_pass<T, N, 0>), g, b, sh, s, a);
       break;
     case 1:
       hipLaunchKernelGGL((k_spec_pass<T, N, 1>), g, b, sh, s, a);
