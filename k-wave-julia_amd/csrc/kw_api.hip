@@ -309,12 +309,13 @@ class Sim final : public SimBase {
     HIPC(hipStreamSynchronize(stream_));
     // ---- residuals and loss
     const long long nres = nsens_ * Nt;
-    DevBuf dobs, dres, dloss, dg, dgref;
+    DevBuf dobs, dres, dloss, dg, dgp, dgref;
     upload(dobs, std::vector<double>(obs, obs + nres));
     dres.alloc(nres * sizeof(double));
-    const unsigned nl = adj_reduce_blocks(nres), ng = adj_reduce_blocks(nvox_);
+    const unsigned nl = adj_reduce_blocks(nres), ng = adj_reduce_blocks(plane_);
     upload(dloss, std::vector<double>(nl, 0.0));
-    upload(dg, std::vector<double>(nvox_, 0.0));
+    upload(dg, std::vector<double>(nvox_, 0.0));    // source and final terms, per stacked voxel
+    upload(dgp, std::vector<double>(plane_, 0.0));  // per-step term, summed over the firings
     upload(dgref, std::vector<double>(ng, 0.0));
     {
       const void* pp[4] = {p_ts_.p, dobs.p, dres.p, dloss.p};
@@ -386,8 +387,8 @@ class Sim final : public SimBase {
         }
         apply_K(Pb.as<T>(), M2d.as<T>(), Q.as<T>(), Qd.as<T>());
         {
-          const void* pp[6] = {c0sq_.p, rs, Q.p, Qd.p, dg.p, dgref.p};
-          HIPC(launch_adj_small<T>(0, stream_, pp, nullptr, nvox_, nd_));
+          const void* pp[6] = {c0sq_.p, rs, Q.p, Qd.p, dgp.p, dgref.p};
+          HIPC(launch_adj_small<T>(0, stream_, pp, nullptr, plane_, nd_, nfire_));
         }
         if (!psrc_idx_h_.empty()) {
           const long long ns = (long long)psrc_idx_h_.size();
@@ -417,13 +418,13 @@ class Sim final : public SimBase {
       HIPC(launch_adj_small<T>(4, stream_, pp, &gterm, nvox_, nd_));
     }
     HIPC(hipStreamSynchronize(stream_));
-    if (nfire_ == 1) {
-      HIPC(hipMemcpy(grad, dg.p, nvox_ * sizeof(double), hipMemcpyDeviceToHost));
-    } else {
-      // sum of the firings' gradients, added in firing order
-      std::vector<double> all(nvox_);
+    {
+      // the per-step term (already summed over the firings) plus each
+      // firing's source and final terms, added in firing order
+      std::vector<double> all(nvox_), gp(plane_);
       HIPC(hipMemcpy(all.data(), dg.p, nvox_ * sizeof(double), hipMemcpyDeviceToHost));
-      for (long long i = 0; i < plane_; ++i) grad[i] = all[i];
+      HIPC(hipMemcpy(gp.data(), dgp.p, plane_ * sizeof(double), hipMemcpyDeviceToHost));
+      for (long long i = 0; i < plane_; ++i) grad[i] = gp[i] + all[i];
       for (int f = 1; f < nfire_; ++f)
         for (long long i = 0; i < plane_; ++i) grad[i] += all[(size_t)f * plane_ + i];
     }

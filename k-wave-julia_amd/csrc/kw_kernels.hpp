@@ -1338,13 +1338,24 @@ __global__ void __launch_bounds__(256) k_r2c_x(const T* f, Cx<T>* S, int n_lines
 // call, so the sums are deterministic (no atomics).
 template <typename T>
 __global__ void __launch_bounds__(256) k_adj_grad(const T* c0sq, const T* rs, const T* Q, const T* Qd, double* g,
-                                                  double* gpart, long long n) {
+                                                  double* gpart, long long plane, int nf) {
+  // Stacked firings share the medium: voxel j of every firing f (j + f plane)
+  // has the same c0, so the firings' terms are summed here and g holds one
+  // grid (each term is read once, g once per grid voxel).
   __shared__ double red[256];
   double acc = 0.0;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const double c2 = (double)c0sq[i], r = (double)rs[i];
-    g[i] += 2.0 * sqrt(c2) * r * (double)Q[i];
-    acc += c2 * r * (double)Qd[i];
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < plane; j += (long long)gridDim.x * blockDim.x) {
+    const double c2 = (double)c0sq[j];
+    double sq = 0.0, sd = 0.0;
+#pragma unroll 4
+    for (int f = 0; f < nf; ++f) {
+      const long long i = j + (long long)f * plane;
+      const double r = (double)rs[i];
+      sq += r * (double)Q[i];
+      sd += r * (double)Qd[i];
+    }
+    g[j] += 2.0 * sqrt(c2) * sq;
+    acc += c2 * sd;
   }
   red[threadIdx.x] = acc;
   __syncthreads();

@@ -87,7 +87,8 @@ template <typename T>
 hipError_t launch_r2c(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const T* f, Cx<T>* S, int n_lines, int nhp,
                       const Cx<T>* tw);
 template <typename T>
-hipError_t launch_adj_small(int which, hipStream_t s, const void* const* p, const double* dargs, long long n, int ndim);
+hipError_t launch_adj_small(int which, hipStream_t s, const void* const* p, const double* dargs, long long n, int ndim,
+                            int nf = 1);
 
 // Elastic stress-velocity passes and the small per-step kernels around them.
 template <typename T>

@@ -33,18 +33,18 @@ hipError_t launch_r2c(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const T* 
   return hipErrorInvalidValue;
 }
 
-// which: 0 grad (c0sq, rs, Q, Qd, g, gpart), 1 src (r0, r1, r2, c0sq, Q, idx, fac, sig, g),
+// which: 0 grad (c0sq, rs, Q, Qd, g, gpart; n = one firing's voxels, nf firings), 1 src (r0, r1, r2, c0sq, Q, idx, fac, sig, g),
 //        2 resid (prec, obs, res, lpart), 3 scatter (Pb, idx, r), 4 final (g, p0, c0sq, r0, r1, r2; dargs[0] = gterm)
 // gpart / lpart hold adj_reduce_blocks(n) per-block partial sums.
 template <typename T>
 hipError_t launch_adj_small(int which, hipStream_t s, const void* const* p, const double* dargs, long long n,
-                            int ndim) {
+                            int ndim, int nf) {
   const unsigned nb = (unsigned)((n + 255) / 256);
   const unsigned nr = adj_reduce_blocks(n);
   switch (which) {
     case 0:
       hipLaunchKernelGGL((k_adj_grad<T>), dim3(nr), dim3(256), 0, s, (const T*)p[0], (const T*)p[1], (const T*)p[2],
-                         (const T*)p[3], (double*)p[4], (double*)p[5], n);
+                         (const T*)p[3], (double*)p[4], (double*)p[5], n, nf);
       break;
     case 1:
       hipLaunchKernelGGL((k_adj_src<T>), dim3(nb), dim3(256), 0, s, (const T*)p[0], (const T*)p[1], (const T*)p[2],
@@ -72,7 +72,7 @@ hipError_t launch_adj_small(int which, hipStream_t s, const void* const* p, cons
 #define KW_INST(T)                                                                                                  \
   template hipError_t launch_adj_pass<T>(int, bool, dim3, dim3, size_t, hipStream_t, const AdjArgs<T>&);           \
   template hipError_t launch_r2c<T>(int, dim3, dim3, size_t, hipStream_t, const T*, Cx<T>*, int, int, const Cx<T>*); \
-  template hipError_t launch_adj_small<T>(int, hipStream_t, const void* const*, const double*, long long, int);
+  template hipError_t launch_adj_small<T>(int, hipStream_t, const void* const*, const double*, long long, int, int);
 KW_INST(float)
 KW_INST(double)
 #undef KW_INST
