@@ -98,6 +98,7 @@ __device__ __forceinline__ Cx<T> el_med(const LineCtx<T, N, V>& L, const T* m, T
 template <typename T, int N, bool CONTIG, int MODE>
 __global__ void __launch_bounds__(512) k_el_pass(ElArgs<T> a) {
   constexpr int V = CONTIG ? 2 : 0;  // contiguous lines: plan family 2 (fewer, longer rows per thread)
+  constexpr bool OPQ = dir_opaque<T, N, CONTIG>();   // as for the acoustic directional passes
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -116,10 +117,10 @@ __global__ void __launch_bounds__(512) k_el_pass(ElArgs<T> a) {
 
   Cx<T> v[E];
   L.load(a.f, v);
-  line_fft<T, N, false, V>(v, lds, tw, L.t);
+  line_fft<T, N, false, V, false, OPQ>(v, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.D[L.n(q)]);
-  line_fft<T, N, true, V>(v, lds, tw, L.t);
+  line_fft<T, N, true, V, false, OPQ>(v, lds, tw, L.t);
 
   if constexpr (MODE == EL_DERIV) {
     if (a.accumulate) {

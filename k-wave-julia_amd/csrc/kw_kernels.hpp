@@ -1214,6 +1214,7 @@ struct AdjArgs {
 template <typename T, int N, bool CONTIG>
 __global__ void __launch_bounds__(512) k_adj_pass(AdjArgs<T> a) {
   constexpr int V = CONTIG ? 1 : 0;
+  constexpr bool OPQ = dir_opaque<T, N, CONTIG>();   // as for the directional passes
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1254,10 +1255,10 @@ __global__ void __launch_bounds__(512) k_adj_pass(AdjArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cscale(rt[q], f);
   }
-  line_fft<T, N, false, V>(v, lds, tw, L.t);
+  line_fft<T, N, false, V, false, OPQ>(v, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dp[L.n(q)]);
-  line_fft<T, N, true, V>(v, lds, tw, L.t);
+  line_fft<T, N, true, V, false, OPQ>(v, lds, tw, L.t);
   // Ut = Ubar + D+(...) ; y = Psg Ut ; Ubar <- Psg y ; z = dt/rho0sg y
   {
     Cx<T> ub[E];
@@ -1284,10 +1285,10 @@ __global__ void __launch_bounds__(512) k_adj_pass(AdjArgs<T> a) {
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cscale(v[q], f);
   }
-  line_fft<T, N, false, V>(v, lds, tw, L.t);
+  line_fft<T, N, false, V, false, OPQ>(v, lds, tw, L.t);
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = cmul(v[q], a.Dm[L.n(q)]);
-  line_fft<T, N, true, V>(v, lds, tw, L.t);
+  line_fft<T, N, true, V, false, OPQ>(v, lds, tw, L.t);
   if (a.accumulate) {
     Cx<T> pb[E];
     L.load(a.Pb, pb);
@@ -1337,18 +1338,37 @@ __global__ void __launch_bounds__(256) k_r2c_x(const T* f, Cx<T>* S, int n_lines
 // c^2 rho_sum Q'.  Each block owns its slot and the grid is the same on every
 // call, so the sums are deterministic (no atomics).
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_grad(const T* c0sq, const T* rs, const T* Q, const T* Qd, double* g,
-                                                  double* gpart, long long plane, int nf) {
+__global__ void __launch_bounds__(256) k_adj_grad(const T* __restrict__ c0sq, const T* __restrict__ rs,
+                                                  const T* __restrict__ Q, const T* __restrict__ Qd,
+                                                  double* __restrict__ g, double* __restrict__ gpart, long long plane,
+                                                  int nf) {
   // Stacked firings share the medium: voxel j of every firing f (j + f plane)
   // has the same c0, so the firings' terms are summed here and g holds one
-  // grid (each term is read once, g once per grid voxel).
+  // grid (each term is read once, g once per grid voxel).  The loads of
+  // eight firings are issued together before they are accumulated.
+  constexpr int B = 8;
   __shared__ double red[256];
   double acc = 0.0;
   for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < plane; j += (long long)gridDim.x * blockDim.x) {
     const double c2 = (double)c0sq[j];
     double sq = 0.0, sd = 0.0;
-#pragma unroll 4
-    for (int f = 0; f < nf; ++f) {
+    int f = 0;
+    for (; f + B <= nf; f += B) {
+      T r[B], q[B], qd[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const long long i = j + (long long)(f + u) * plane;
+        r[u] = rs[i];
+        q[u] = Q[i];
+        qd[u] = Qd[i];
+      }
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        sq += (double)r[u] * (double)q[u];
+        sd += (double)r[u] * (double)qd[u];
+      }
+    }
+    for (; f < nf; ++f) {
       const long long i = j + (long long)f * plane;
       const double r = (double)rs[i];
       sq += r * (double)Q[i];
