@@ -47,6 +47,7 @@ def kernel_bytes_per_voxel(name: str, nx: int, s: int = 4) -> float:
         "k_spec_pass_z_mul": 2 * half,
         "k_spec_pass_y_inv": 2 * half,
         "k_c2r_x": half + s,
+        "k_plane_c2r": half + s,
     }.get(name, 0.0)
 
 
@@ -60,6 +61,7 @@ KERNELS = {
     "k_spec_pass<float,256,2> (spectral z fwd*M*inv)": ("k_spec_pass_z_mul",),
     "k_spec_pass<float,256,1> (spectral y inverse)": ("k_spec_pass_y_inv",),
     "k_c2r_x<float,256> (c2r x pass)": ("k_c2r_x",),
+    "k_plane_c2r<float,256,256> (y inverse + c2r per z plane)": ("k_plane_c2r",),
 }
 
 

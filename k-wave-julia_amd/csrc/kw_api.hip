@@ -244,8 +244,9 @@ class Sim final : public SimBase {
     long long done = 0;
     for (; done < steps && t_ < Nt_; ++done) step(&pr);
     HIPC(hipStreamSynchronize(stream_));
-    static const int cls_of[kNumClasses] = {TC_DIR_X, TC_DIR_Y,  TC_DIR_Z,  TC_SPEC_Y, TC_SPEC_Z,
-                                            TC_SPEC_Y, TC_SPEC_Y, TC_C2R,    -1,        -1};
+    static const int cls_of[kNumClasses] = {TC_DIR_X, TC_DIR_Y,  TC_DIR_Z, TC_SPEC_Y, TC_SPEC_Z, TC_SPEC_Y,
+                                            TC_SPEC_Y, TC_C2R,    -1,       -1,        -1};
+    static_assert(kNumClasses == 11, "one tuning class (or -1) per pass class");
     for (int c = 0; c < TC_COUNT; ++c) ms[c] = 0.0;
     for (auto& r : pr.ev) {
       float e = 0.f;
@@ -514,10 +515,11 @@ class Sim final : public SimBase {
 
   template <typename U>
   friend class Batch;
-  enum Cls { C_DIR_X, C_DIR_Y, C_DIR_Z, C_SPEC_Y_FWD, C_SPEC_Z, C_SPEC_Y_INV, C_SPEC_Y, C_C2R, C_REC, C_TR, kNumClasses };
+  enum Cls { C_DIR_X, C_DIR_Y, C_DIR_Z, C_SPEC_Y_FWD, C_SPEC_Z, C_SPEC_Y_INV, C_SPEC_Y, C_C2R, C_REC, C_TR,
+             C_PLANE_C2R, kNumClasses };
   static constexpr const char* kClassNames[kNumClasses] = {
       "k_dir_pass_x_eos", "k_dir_pass_y", "k_dir_pass_z", "k_spec_pass_y_fwd", "k_spec_pass_z_mul",
-      "k_spec_pass_y_inv", "k_spec_pass_y_mul", "k_c2r_x", "k_record", "k_tr_inject"};
+      "k_spec_pass_y_inv", "k_spec_pass_y_mul", "k_c2r_x", "k_record", "k_tr_inject", "k_plane_c2r"};
   struct ProfRec {
     int cls;
     hipEvent_t a, b;
@@ -1058,6 +1060,14 @@ class Sim final : public SimBase {
     ca_.tw = tw_[0].as<Cx<T>>();
     ca_.p = p_.as<T>();
     ca_.n_lines = (int)(N_[1] * N_[2]);
+    // the fused y-inverse / c2r plane pass where the plane fits a workgroup
+    plane_c2r_ = nd_ == 3 && !batch_mode_ && !bs_[0] && !bs_[1] && plane_c2r_supported<T>((int)N_[0], (int)N_[1]);
+    std::memset(&plane_args_, 0, sizeof(plane_args_));
+    plane_args_.S1 = S1_.as<Cx<T>>();
+    plane_args_.tw_x = tw_[0].as<Cx<T>>();
+    plane_args_.tw_y = nd_ > 1 ? tw_[1].as<Cx<T>>() : nullptr;
+    plane_args_.p = p_.as<T>();
+    plane_args_.nhp = (int)NHP_;
     ca_.nhp = (int)NHP_;
     ca_.nl = (int)N_[0];
     ca_.ch = chirp(0);
@@ -1233,15 +1243,21 @@ class Sim final : public SimBase {
       SpecArgs<T> Z = sa_[1];
       Z.mode = 2;
       launch(pr, C_SPEC_Z, [&] { HIPC(lspec(2, spec_[1], Z)); });
-      Y.mode = 1;
-      Y.S2 = nullptr;
-      launch(pr, C_SPEC_Y_INV, [&] { HIPC(lspec(1, spec_[0], Y)); });
+      if (plane_c2r_) {   // y inverse and c2r fused per z plane
+        launch(pr, C_PLANE_C2R, [&] {
+          HIPC(launch_plane_c2r<T>((int)N_[0], (int)N_[1], (unsigned)N_[2], stream_, plane_args_));
+        });
+      } else {
+        Y.mode = 1;
+        Y.S2 = nullptr;
+        launch(pr, C_SPEC_Y_INV, [&] { HIPC(lspec(1, spec_[0], Y)); });
+      }
     } else if (nd_ == 2) {
       SpecArgs<T> Y = sa_[0];
       Y.mode = 2;
       launch(pr, C_SPEC_Y, [&] { HIPC(lspec(1, spec_[0], Y)); });
     }
-    launch(pr, C_C2R, [&] { HIPC(lc2r(c2r_, ca_)); });
+    if (!plane_c2r_) launch(pr, C_C2R, [&] { HIPC(lc2r(c2r_, ca_)); });
     if (nsens_ > 0 && !cartesian_ && rec_ && !no_record_) {
       ra_.t = t;
       launch(pr, C_REC, [&] {
@@ -1318,6 +1334,8 @@ class Sim final : public SimBase {
   bool bs_[3] = {false, false, false};   // axis runs chirp-z (Bluestein) lines
   DevBuf chc_[3], chf_[3], chb_[3];       // chirp tables of those axes
   int nfire_ = 1;          // stacked firings (2-D), 1 otherwise
+  bool plane_c2r_ = false; // 3-D step: y inverse and c2r fused per z plane
+  PlaneArgs<T> plane_args_{};
   long long plane_ = 0;    // voxels of one firing's grid
   hipStream_t stream_ = nullptr;
   int nd_ = 0;

@@ -316,6 +316,12 @@ struct Plan<256, 2> {
   using R = Radices<8, 8, 4>;
 };
 
+// V = 5: the lines of the fused y-inverse / complex-to-real plane kernel: the
+// plans of family 2, with twiddles read from the table (the kernel's
+// registers hold a whole plane of the spectrum).
+template <int N>
+struct Plan<N, 5> : Plan<N, 2> {};
+
 // V = 4: contiguous-line passes (x pass, complex-to-real) of chirp-z lines,
 // which carry the chirp and filter arrays on top of the field arrays: the long
 // plans use 8 elements per thread (the 16-element plans spill registers there).
@@ -378,7 +384,7 @@ struct TwCount<N, V, Radices<Rs...>> {
 template <typename T, int N, int V = 0>
 struct Twiddles {
   static constexpr int TOT = TwCount<N, V, typename Plan<N, V>::R>::value;
-  static constexpr bool REG = TOT > 0 && TOT <= 16;
+  static constexpr bool REG = V != 5 && TOT > 0 && TOT <= 16;
   Cx<T> w[REG ? TOT : 1];
   const Cx<T>* g;
 };

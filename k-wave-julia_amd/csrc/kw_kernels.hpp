@@ -926,6 +926,127 @@ __global__ void __launch_bounds__(256) k_c2r_x_batch(const C2RArgs<T>* __restric
 }
 
 // ---------------------------------------------------------------------------
+// y inverse + complex-to-real x pass fused per z plane.  One workgroup holds
+// a whole NX x NY plane of the half spectrum: the NX/2 + 1 columns go to
+// NX/2 column slots (the DC and Nyquist columns share slot 0 as real and
+// imaginary part: after the y and z transforms both are Hermitian in ky, so
+// their y inverses are real and one complex transform carries both), each
+// thread keeps its columns' y inverses in registers, and rows move to the
+// c2r transforms through LDS in chunks.  The plane is read once and p
+// written once, instead of the spectrum's extra round trip of the separate
+// y inverse pass.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct PlaneArgs {
+  const Cx<T>* S1;    // half spectrum: rows of nhp bins, planes of nhp * NY
+  const Cx<T>* tw_x;  // line twiddle tables of the x and y axes
+  const Cx<T>* tw_y;
+  T* p;
+  int nhp;
+};
+
+template <typename T, int NX, int NY>
+struct PlaneGeom {
+  static constexpr int NT = 1024;                                  // threads
+  static constexpr int NC = NX / 2;                                // column slots
+  static constexpr int TY = Plan<NY, 5>::TPL, EY = Plan<NY, 5>::E;
+  static constexpr int TX = Plan<NX, 5>::TPL, EX = Plan<NX, 5>::E;
+  static constexpr int CPR = NT / TY;                              // columns per round
+  static constexpr int R = NC / CPR;                               // rounds
+  static constexpr int RPC = 2 * (NT / TX);                        // rows per chunk
+  static constexpr int NCH = NY / RPC;                             // chunks
+  static constexpr int CP = NC + 1;                                // chunk row pitch (bins)
+  static constexpr int XCH = (CPR * LdsLayout<NY>::PITCH > (NT / TX) * LdsLayout<NX>::PITCH)
+                                 ? CPR * LdsLayout<NY>::PITCH
+                                 : (NT / TX) * LdsLayout<NX>::PITCH;   // exchange slices
+  static constexpr size_t LDS = (size_t)(XCH + RPC * CP) * sizeof(Cx<T>);
+  static_assert(NC % CPR == 0 && NY % RPC == 0 && NY == TY * EY && RPC % TY == 0, "plane geometry");
+};
+
+template <typename T, int NX, int NY>
+__global__ void __launch_bounds__(1024) k_plane_c2r(PlaneArgs<T> a) {
+  using G = PlaneGeom<T, NX, NY>;
+  constexpr int NC = G::NC, TY = G::TY, EY = G::EY, TX = G::TX, EX = G::EX, CPR = G::CPR, R = G::R;
+  constexpr int RPC = G::RPC, NCH = G::NCH, CP = G::CP;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  Cx<T>* xch = reinterpret_cast<Cx<T>*>(smem_raw);
+  Cx<T>* chunk = xch + G::XCH;
+  const int tid = threadIdx.x;
+  const int c = tid % CPR, t = tid / CPR;   // column slot in a round, thread in the column
+  const Cx<T>* __restrict__ S = a.S1 + (long long)blockIdx.x * NY * a.nhp;
+
+  // every column of the plane (rows t + TY q): round 0 with the Nyquist
+  // column folded into slot 0 first, then the other rounds, whose loads are
+  // in flight during round 0's transform
+  Cx<T> v[R][EY];
+#pragma unroll
+  for (int q = 0; q < EY; ++q) v[0][q] = at32(S, (unsigned)((t + TY * q) * a.nhp + c));
+  if (c == 0) {   // slot 0: DC + i Nyquist
+#pragma unroll
+    for (int q = 0; q < EY; ++q) {
+      const Cx<T> b = at32(S, (unsigned)((t + TY * q) * a.nhp + NC));
+      v[0][q] = Cx<T>{v[0][q].x - b.y, v[0][q].y + b.x};
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int r = 1; r < R; ++r)
+#pragma unroll
+    for (int q = 0; q < EY; ++q) v[r][q] = at32(S, (unsigned)((t + TY * q) * a.nhp + r * CPR + c));
+  {
+    const Twiddles<T, NY, 5> tw = load_twiddles<T, NY, 5>(a.tw_y, t);
+#pragma unroll
+    for (int r = 0; r < R; ++r) line_fft<T, NY, true, 5, false, true>(v[r], xch + c * LdsLayout<NY>::PITCH, tw, t);
+  }
+
+  // c2r of row pairs, RPC rows at a time
+  const int pr = tid / TX, tx = tid % TX;
+  const Twiddles<T, NX, 5> twx = load_twiddles<T, NX, 5>(a.tw_x, tx);
+  T* __restrict__ P = a.p + (long long)blockIdx.x * NY * NX;
+#pragma unroll
+  for (int j = 0; j < NCH; ++j) {
+    // this thread's values of the chunk's rows: q = j * RPC / TY ... (rows t + TY q)
+#pragma unroll
+    for (int qq = 0; qq < RPC / TY; ++qq) {
+      const int q = j * (RPC / TY) + qq;
+      const int row = t + TY * qq;   // row within the chunk
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        if (r == 0 && c == 0) {
+          chunk[row * CP] = Cx<T>{v[0][q].x, (T)0};
+          chunk[row * CP + NC] = Cx<T>{v[0][q].y, (T)0};
+        } else {
+          chunk[row * CP + r * CPR + c] = v[r][q];
+        }
+      }
+    }
+    __syncthreads();
+    Cx<T> z[EX];
+#pragma unroll
+    for (int q = 0; q < EX; ++q) {
+      const int k = tx + TX * q;
+      const bool mir = k > NC;
+      const int kk = mir ? NX - k : k;
+      Cx<T> A = chunk[(2 * pr) * CP + kk], B = chunk[(2 * pr + 1) * CP + kk];
+      if (mir) {
+        A.y = -A.y;
+        B.y = -B.y;
+      }
+      z[q] = Cx<T>{A.x - B.y, A.y + B.x};
+    }
+    line_fft<T, NX, true, 5, wave_local_lines<NX, 5>(), true>(z, xch + pr * LdsLayout<NX>::PITCH, twx, tx);
+    const int ra = j * RPC + 2 * pr;
+#pragma unroll
+    for (int q = 0; q < EX; ++q) {
+      const int n = tx + TX * q;
+      at32(P, (unsigned)(ra * NX + n)) = z[q].x;
+      at32(P, (unsigned)((ra + 1) * NX + n)) = z[q].y;
+    }
+    __syncthreads();   // the next chunk overwrites the chunk rows and exchange slices
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Sensor recording (record_sensor_data!, first_order_steps.jl:728-794)
 // ---------------------------------------------------------------------------
 enum : unsigned {

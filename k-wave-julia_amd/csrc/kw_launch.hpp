@@ -43,6 +43,12 @@ template <typename T>
 hipError_t launch_spec(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<T>& a);
 template <typename T>
 hipError_t launch_c2r(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const C2RArgs<T>& a);
+// Fused y inverse + complex-to-real pass, one workgroup per z plane of nx x ny
+// (plane_c2r_supported: the plane sizes and precisions it is compiled for).
+template <typename T>
+bool plane_c2r_supported(int nx, int ny);
+template <typename T>
+hipError_t launch_plane_c2r(int nx, int ny, unsigned nz, hipStream_t s, const PlaneArgs<T>& a);
 // Bluestein forms (n_plan from bluestein_plan; the args carry the line length and chirp tables).
 template <typename T>
 hipError_t launch_dir_bs(int n_plan, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const PassArgs<T>& a);

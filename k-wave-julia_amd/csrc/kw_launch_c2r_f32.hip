@@ -16,4 +16,19 @@ hipError_t launch_c2r<float>(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, co
   return hipErrorInvalidValue;
 }
 
+template <>
+bool plane_c2r_supported<float>(int nx, int ny) {
+  return nx == 256 && ny == 256;
+}
+
+template <>
+hipError_t launch_plane_c2r<float>(int nx, int ny, unsigned nz, hipStream_t s, const PlaneArgs<float>& a) {
+  if (nx == 256 && ny == 256) {
+    using G = PlaneGeom<float, 256, 256>;
+    hipLaunchKernelGGL((k_plane_c2r<float, 256, 256>), dim3(nz), dim3(G::NT), G::LDS, s, a);
+    return hipGetLastError();
+  }
+  return hipErrorInvalidValue;
+}
+
 }  // namespace kw

@@ -16,4 +16,15 @@ hipError_t launch_c2r<double>(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, c
   return hipErrorInvalidValue;
 }
 
+// The FP64 plane does not fit a workgroup's registers: no fused form.
+template <>
+bool plane_c2r_supported<double>(int, int) {
+  return false;
+}
+
+template <>
+hipError_t launch_plane_c2r<double>(int, int, unsigned, hipStream_t, const PlaneArgs<double>&) {
+  return hipErrorInvalidValue;
+}
+
 }  // namespace kw
