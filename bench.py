@@ -402,6 +402,95 @@ def bench_elastic(a):
     print(json.dumps(out))
 
 
+def bench_fft(a):
+    """FFT-component micro-benchmark, as run_benchmarks.jl:328-393 times its FFT plans.
+
+    GPU: the line-FFT derivative along each axis (forward line transform, the
+    staggered i k multiplier, inverse transform: the spectral_gradient! operator,
+    fft_utils.jl:126-182) on FP32 device arrays through the allocation-free entry
+    point, on torch's current stream, HIP events around `--steps` calls after 3
+    untimed ones; per call the time, the algorithmic bytes (the field read and the
+    derivative written, 8 B per voxel) over it, and 2 x 5 N log2 N flop per line.
+    CPU: the reference's own sizes (1-D rfft single-threaded, 2-D and 3-D rfftn on
+    every core) with scipy.fft in FP64 as the FFTW stand-in, median of repeated runs.
+    """
+    import ctypes as C
+    import numpy as np
+    import torch
+    from kwave_amd import _lib
+
+    lib = _lib.load()
+    steps = a.steps if a.steps != 1200 else 20
+    stream = torch.cuda.current_stream()
+    gpu = []
+    for shape in [(256,), (1024,), (4096,), (256, 256), (512, 512), (1024, 1024), (64, 64, 64), (128, 128, 128),
+                  (256, 256, 256), (512, 512, 512)]:
+        nd = len(shape)
+        n = int(np.prod(shape))
+        N = (C.c_int64 * 3)(*(list(shape) + [1] * (3 - nd)))
+        dd = (C.c_double * 3)(*([1e-4] * 3))
+        f = torch.randn(n, dtype=torch.float32, device="cuda")
+        o = torch.empty_like(f)
+
+        def call(axis):
+            _lib.check(lib.kw_spectral_gradient_noalloc(_lib.context(0), 4, nd, N, dd, axis, 1,
+                                                        C.c_void_p(f.data_ptr()), C.c_void_p(o.data_ptr()),
+                                                        C.c_void_p(stream.cuda_stream)))
+        for axis in range(nd):
+            for _ in range(3):
+                call(axis)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            for _ in range(steps):
+                call(axis)
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1) / steps
+            flops = 2 * 5 * n * np.log2(shape[axis])
+            gpu.append({"shape": list(shape), "axis": axis, "us": round(ms * 1e3, 2),
+                        "GB_s": round(8 * n / (ms * 1e-3) / 1e9, 1), "GFLOP_s": round(flops / (ms * 1e-3) / 1e9, 1)})
+            print(f"[fft] GPU derivative {shape} axis {axis}: {ms * 1e3:8.2f} us  {gpu[-1]['GB_s']:7.1f} GB/s",
+                  file=sys.stderr, flush=True)
+        del f, o
+        torch.cuda.empty_cache()
+    cpu = []
+    if not a.no_cpu:
+        import scipy.fft as sf
+        nproc = len(os.sched_getaffinity(0))
+        sizes = [((n,), 1) for n in (128, 256, 512, 1024, 2048, 4096, 8192, 16384)]
+        sizes += [((n, n), nproc) for n in (64, 128, 256, 512, 1024)]
+        sizes += [((n, n, n), nproc) for n in (32, 64, 128, 256)]
+        rng = np.random.default_rng(42)
+        for shape, workers in sizes:
+            x = rng.random(shape)
+            sf.rfftn(x, workers=workers)
+            reps = 200 if len(shape) == 1 else (20 if np.prod(shape) <= 1 << 20 else 5)
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                sf.rfftn(x, workers=workers)
+                ts.append(time.perf_counter() - t0)
+            med = float(np.median(ts))
+            nn = int(np.prod(shape))
+            cpu.append({"shape": list(shape), "workers": workers, "median_us": round(med * 1e6, 2),
+                        "GFLOP_s": round(5 * nn * np.log2(nn) / med / 1e9, 2)})
+    big = [r for r in gpu if r["shape"] == [256, 256, 256]]
+    val = sum(r["GB_s"] for r in big) / len(big)
+    print(json.dumps({
+        "metric": "line-FFT derivative throughput (FFT micro-benchmark)", "value": round(val, 1), "unit": "GB/s",
+        "n_gpus": 1, "steps": steps, "warmup": 3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "spectral derivative (forward line FFT, staggered i k, inverse) of a 256^3 FP32 "
+                               "field along x, y and z, mean over the axes; other sizes in gpu_rows",
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(val, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(val / HBM_PEAK_GBS, 3), "traffic": None},
+        "gpu_rows": gpu,
+        "cpu_fft_rows": cpu,
+        "cpu_fft_note": "scipy.fft.rfftn FP64 (FFTW stand-in) on the reference's micro-benchmark sizes",
+    }))
+
+
 def copy_probe():
     """A plain elementwise copy of known size (1 GiB read + 1 GiB written per launch), for
     calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE against algorithmic bytes."""
@@ -426,7 +515,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 accuracy run")
     ap.add_argument("--accuracy-nt", type=int, default=1200,
                     help="steps of the FP32-vs-FP64 accuracy runs (the workload's Nt)")
-    ap.add_argument("--config", choices=["c2", "batch", "adjoint", "c4", "elastic"], default="c2")
+    ap.add_argument("--config", choices=["c2", "batch", "adjoint", "c4", "elastic", "fft"], default="c2")
     ap.add_argument("--firings", type=int, default=8)
     ap.add_argument("--fd-pixels", type=int, default=32)
     ap.add_argument("--c4-maps", choices=["fp16", "fp32"], default="fp16")
@@ -453,6 +542,8 @@ def main():
         if a.steps == 1200:
             a.steps = 200
         return bench_elastic(a)
+    if a.config == "fft":
+        return bench_fft(a)
 
     import numpy as np
     import kwave_amd as K
