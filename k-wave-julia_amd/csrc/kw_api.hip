@@ -1061,7 +1061,7 @@ class Sim final : public SimBase {
     ca_.p = p_.as<T>();
     ca_.n_lines = (int)(N_[1] * N_[2]);
     // the fused y-inverse / c2r plane pass where the plane fits a workgroup
-    plane_c2r_ = nd_ == 3 && !batch_mode_ && !bs_[0] && !bs_[1] && plane_c2r_supported<T>((int)N_[0], (int)N_[1]);
+    plane_c2r_ = nd_ == 3 && !batch_mode_ && !bs_[0] && !bs_[1] && plane_c2r_supported<T>((int)N_[0], (int)N_[1], lds_max_);
     std::memset(&plane_args_, 0, sizeof(plane_args_));
     plane_args_.S1 = S1_.as<Cx<T>>();
     plane_args_.tw_x = tw_[0].as<Cx<T>>();

@@ -44,9 +44,10 @@ hipError_t launch_spec(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const Sp
 template <typename T>
 hipError_t launch_c2r(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const C2RArgs<T>& a);
 // Fused y inverse + complex-to-real pass, one workgroup per z plane of nx x ny
-// (plane_c2r_supported: the plane sizes and precisions it is compiled for).
+// (plane_c2r_supported: the plane sizes and precisions it is compiled for,
+// given the workgroup LDS limit of the device).
 template <typename T>
-bool plane_c2r_supported(int nx, int ny);
+bool plane_c2r_supported(int nx, int ny, size_t lds_max);
 template <typename T>
 hipError_t launch_plane_c2r(int nx, int ny, unsigned nz, hipStream_t s, const PlaneArgs<T>& a);
 // Bluestein forms (n_plan from bluestein_plan; the args carry the line length and chirp tables).

@@ -17,8 +17,8 @@ hipError_t launch_c2r<float>(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, co
 }
 
 template <>
-bool plane_c2r_supported<float>(int nx, int ny) {
-  return nx == 256 && ny == 256;
+bool plane_c2r_supported<float>(int nx, int ny, size_t lds_max) {
+  return nx == 256 && ny == 256 && PlaneGeom<float, 256, 256>::LDS <= lds_max;
 }
 
 template <>

@@ -18,7 +18,7 @@ hipError_t launch_c2r<double>(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, c
 
 // The FP64 plane does not fit a workgroup's registers: no fused form.
 template <>
-bool plane_c2r_supported<double>(int, int) {
+bool plane_c2r_supported<double>(int, int, size_t) {
   return false;
 }
 
