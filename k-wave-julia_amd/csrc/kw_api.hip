@@ -1537,14 +1537,10 @@ class Batch final : public BatchBase {
         g.n = n;
         std::vector<int> wg_sim, wg_first(sims_.size(), 0);
         std::vector<unsigned char> args;
-        int idx = 0;
         for (size_t i = 0; i < sims_.size(); ++i) {
           Sim<T>& s = *sims_[i];
           const PassCfg& c = cfg_of(s, k);
-          if (c.n != n) {
-            ++idx;
-            continue;
-          }
+          if (c.n != n) continue;
           if (g.nwg > 0 && (c.block.x != g.block.x || c.shmem != g.shmem))
             raise(KW_ERR_STATE, "batched simulations disagree on the workgroup shape of a line length");
           g.block = c.block;
@@ -1555,7 +1551,6 @@ class Batch final : public BatchBase {
           for (unsigned w = 0; w < c.grid.x; ++w) wg_sim.push_back((int)i);
           g.nwg += c.grid.x;
           g.two = g.two || s.stokes_;
-          ++idx;
         }
         // argument records for every simulation (indexed by simulation)
         for (auto& sp : sims_) append_args(*sp, k, args);
