@@ -227,12 +227,17 @@ int32_t kw_sim_profile(kw_sim* sim, int64_t n_steps, int32_t max_classes, int32_
  * the workgroup LDS) for every pass and keeps
  * the fastest per (precision, pass, line length) in a process-wide table
  * that later kw_sim_create / kw_batch_create calls read.  With n_sims == 1
- * the trials run descs[0] as a single simulation; with n_sims > 1 they run
- * the whole array as one batch (kw_batch_create), so the table fits that
- * batch.
+ * the trials run descs[0] as a single simulation and, for passes whose line
+ * length is 256 or 512, also time the alternative plan families 0, 1, 2
+ * (radix schedule and threads per line), which single simulations then use;
+ * with n_sims > 1 they run the whole array as one batch (kw_batch_create),
+ * so the table fits that batch (batches keep the default plan families).
  * kw_tune_export writes the table as text lines "precision pass n lines"
- * (*len = its length; buf gets at most cap-1 bytes plus a NUL);
- * kw_tune_import merges such text; kw_tune_clear empties the table.
+ * with a fifth column "plan" where a plan family other than the pass's
+ * default was chosen (*len = its length; buf gets at most cap-1 bytes plus
+ * a NUL); kw_tune_import merges such text (a line without the fifth column
+ * resets that entry to the default family; KW_ERR_INVALID for a plan family
+ * the line length does not have); kw_tune_clear empties the table.
  */
 int32_t kw_autotune(kw_ctx* ctx, const kw_sim_desc* descs, int32_t n_sims, int32_t steps);
 int32_t kw_tune_export(char* buf, int64_t cap, int64_t* len);

@@ -23,17 +23,30 @@ using namespace kwh;
 namespace {
 
 // ---------------------------------------------------------------------------
-// Launch-shape tuning table: lines per workgroup for each (precision bytes,
-// pass class, line length), filled by kw_autotune's timed trials or by
-// kw_tune_import; read when a simulation or batch is created.
+// Launch-shape tuning table: lines per workgroup and plan family for each
+// (precision bytes, pass class, line length), filled by kw_autotune's timed
+// trials or by kw_tune_import; read when a simulation or batch is created.
 // ---------------------------------------------------------------------------
 enum TuneClass { TC_DIR_X = 0, TC_DIR_Y, TC_DIR_Z, TC_SPEC_Y, TC_SPEC_Z, TC_C2R, TC_COUNT };
 std::mutex g_tune_mu;
-std::map<std::array<int, 3>, int> g_tune;
+std::map<std::array<int, 3>, int> g_tune;      // lines per workgroup
+std::map<std::array<int, 3>, int> g_tune_pv;   // plan family (absent: the default)
 // Inside an autotuner trial: lines per workgroup = default >> g_tune_shift
-// (shift -1 doubles it, up to the pass's maximum); kNoTrial outside trials.
+// (shift -1 doubles it, up to the pass's maximum) and plan family
+// g_tune_plan where the line length has it (-1: defaults); kNoTrial outside
+// trials.
 constexpr int kNoTrial = 99;
 thread_local int g_tune_shift = kNoTrial;
+thread_local int g_tune_plan = kNoTrial;
+
+// Plan family of a single simulation's pass: the trial's or the table's
+// choice where the line length has alternatives, else -1 (the default).
+int tuned_plan(int prec, int cls, int n) {
+  if (g_tune_shift != kNoTrial) return plan_alternative(n, g_tune_plan) ? g_tune_plan : -1;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune_pv.find({prec, cls, n});
+  return it != g_tune_pv.end() && plan_alternative(n, it->second) ? it->second : -1;
+}
 
 // Lines per workgroup of a pass: the tuning table's entry for (precision,
 // class, line length) or the trial's choice, clamped to [1, maxl] (maxl: the
@@ -237,7 +250,7 @@ class Sim final : public SimBase {
   // Autotuner trial: `warm` untimed steps, then per-class device time summed
   // over `steps` timed steps (ms per step), with the lines per workgroup and
   // the line length each class ran with (n = 0: class absent).
-  void trial(long long warm, long long steps, double* ms, int* lp, int* nn) {
+  void trial(long long warm, long long steps, double* ms, int* lp, int* nn, int* pv) {
     HIPC(hipSetDevice(dev_));
     for (long long i = 0; i < warm && t_ < Nt_; ++i) step(nullptr);
     Prof pr;
@@ -261,6 +274,7 @@ class Sim final : public SimBase {
     for (int c = 0; c < TC_COUNT; ++c) {
       lp[c] = cf[c] ? cf[c]->req : 0;
       nn[c] = cf[c] ? cf[c]->n : 0;
+      pv[c] = cf[c] ? cf[c]->pv : -1;
     }
   }
 
@@ -400,7 +414,8 @@ class Sim final : public SimBase {
         }
         for (int a = 0; a < nd_; ++a) {
           aa[a].accumulate = a > 0 ? 1 : 0;
-          HIPC(launch_adj_pass<T>(cfg_[a].n, a == 0, cfg_[a].grid, cfg_[a].block, cfg_[a].shmem, stream_, aa[a]));
+          HIPC(launch_adj_pass<T>(cfg_[a].n, a == 0, cfg_[a].grid, cfg_[a].block, cfg_[a].shmem, stream_, aa[a],
+                                  cfg_[a].pv));
         }
       }
     }
@@ -444,28 +459,28 @@ class Sim final : public SimBase {
   // Q = K f and Qd = dK/dc_ref f for a real field f (K = F^-1 M1 F)
   void apply_K(const T* f, const T* M2, T* Q, T* Qd) {
     HIPC(launch_r2c<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, f, S1_.as<Cx<T>>(), (int)(N_[1] * N_[2]),
-                       (int)NHP_, tw_[0].as<Cx<T>>()));
+                       (int)NHP_, tw_[0].as<Cx<T>>(), c2r_.pv));
     if (!S2_.p) S2_.alloc((size_t)NHP_ * N_[1] * N_[2] * sizeof(Cx<T>));
     if (nd_ == 3) {
       SpecArgs<T> Y = sa_[0];
       Y.mode = 0;
       Y.S2 = nullptr;
-      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y));
+      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y, spec_[0].pv));
       SpecArgs<T> Z = sa_[1];
       Z.mode = 5;
       Z.S2 = S2_.as<Cx<T>>();
       Z.M2 = M2;
-      HIPC(launch_spec<T>(spec_[1].n, spec_[1].grid, spec_[1].block, spec_[1].shmem, stream_, Z));
+      HIPC(launch_spec<T>(spec_[1].n, spec_[1].grid, spec_[1].block, spec_[1].shmem, stream_, Z, spec_[1].pv));
       Y.mode = 1;
-      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y));
+      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y, spec_[0].pv));
       Y.S1 = S2_.as<Cx<T>>();
-      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y));
+      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y, spec_[0].pv));
     } else if (nd_ == 2) {
       SpecArgs<T> Y = sa_[0];
       Y.mode = 5;
       Y.S2 = S2_.as<Cx<T>>();
       Y.M2 = M2;
-      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y));
+      HIPC(launch_spec<T>(spec_[0].n, spec_[0].grid, spec_[0].block, spec_[0].shmem, stream_, Y, spec_[0].pv));
     }
     C2RArgs<T> c = ca_;
     c.S2 = nullptr;
@@ -475,17 +490,17 @@ class Sim final : public SimBase {
     if (nd_ == 1) {
       // 1-D: the multipliers are applied in the c2r pass
       HIPC(hipMemcpyAsync(S2_.p, S1_.p, (size_t)NHP_ * sizeof(Cx<T>), hipMemcpyDeviceToDevice, stream_));
-      HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c));
+      HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c, c2r_.pv));
       c.S1 = S2_.as<Cx<T>>();
       c.M1 = M2;
       c.p = Qd;
-      HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c));
+      HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c, c2r_.pv));
       return;
     }
-    HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c));
+    HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c, c2r_.pv));
     c.S1 = S2_.as<Cx<T>>();
     c.p = Qd;
-    HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c));
+    HIPC(launch_c2r<T>(c2r_.n, c2r_.grid, c2r_.block, c2r_.shmem, stream_, c, c2r_.pv));
   }
 
   // d/dc_ref [kappa (1 + nabla1)] / N on the octant (heterogeneous branch)
@@ -684,8 +699,12 @@ class Sim final : public SimBase {
   PassCfg pass_cfg(long long n, bool contig, long long lines, long long outer, int cls, bool bs = false) {
     PassCfg c;
     c.n = (int)n;
-    // plan family: 0 strided, 1 contiguous, 2 complex-to-real, 4 contiguous chirp-z lines
-    const int tpl = plan_tpl((int)n, contig && bs ? 4 : (cls == TC_C2R ? 2 : (contig ? 1 : 0)));
+    // plan family: 0 strided, 1 contiguous, 2 complex-to-real, 4 contiguous chirp-z lines, or a tuned
+    // alternative (single simulations on compiled lengths; batch kernels take the defaults)
+    const int dflt_v = contig && bs ? 4 : (cls == TC_C2R ? 2 : (contig ? 1 : 0));
+    c.pv = (bs || batch_mode_) ? -1 : tuned_plan((int)sizeof(T), cls, (int)n);
+    if (c.pv == dflt_v) c.pv = -1;   // the default family: its own kernels
+    const int tpl = plan_tpl((int)n, c.pv >= 0 ? c.pv : dflt_v);
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
     // strided lines: at least 16 line pairs per workgroup (128-byte rows in FP32)
     // when that fits 512 threads; contiguous lines: 256 threads
@@ -721,7 +740,9 @@ class Sim final : public SimBase {
   PassCfg spec_cfg(long long n, long long outer, int cls) {
     PassCfg c;
     c.n = (int)n;
-    const int tpl = plan_tpl((int)n);
+    c.pv = (bs_[cls == TC_SPEC_Y ? 1 : 2] || batch_mode_) ? -1 : tuned_plan((int)sizeof(T), cls, (int)n);
+    if (c.pv == 0) c.pv = -1;   // the default family: its own kernels
+    const int tpl = plan_tpl((int)n, c.pv >= 0 ? c.pv : 0);
     const size_t line_bytes = (size_t)plan_pitch((int)n) * sizeof(Cx<T>);
     int lpw = std::max(256, std::min(512, 16 * tpl)) / tpl;   // as for strided passes
     while (lpw > 1 && (size_t)lpw * line_bytes > lds_max_) lpw /= 2;
@@ -1189,15 +1210,15 @@ class Sim final : public SimBase {
   // pass launches on axis a: the compiled-length kernels or their chirp-z form
   hipError_t ldir(int a, const PassCfg& c, const PassArgs<T>& A) {
     return bs_[a] ? launch_dir_bs<T>(c.n, a == 0, c.grid, c.block, c.shmem, stream_, A)
-                  : launch_dir<T>(c.n, a == 0, c.grid, c.block, c.shmem, stream_, A);
+                  : launch_dir<T>(c.n, a == 0, c.grid, c.block, c.shmem, stream_, A, c.pv);
   }
   hipError_t lspec(int a, const PassCfg& c, const SpecArgs<T>& A) {
     return bs_[a] ? launch_spec_bs<T>(c.n, c.grid, c.block, c.shmem, stream_, A)
-                  : launch_spec<T>(c.n, c.grid, c.block, c.shmem, stream_, A);
+                  : launch_spec<T>(c.n, c.grid, c.block, c.shmem, stream_, A, c.pv);
   }
   hipError_t lc2r(const PassCfg& c, const C2RArgs<T>& A) {
     return bs_[0] ? launch_c2r_bs<T>(c.n, c.grid, c.block, c.shmem, stream_, A)
-                  : launch_c2r<T>(c.n, c.grid, c.block, c.shmem, stream_, A);
+                  : launch_c2r<T>(c.n, c.grid, c.block, c.shmem, stream_, A, c.pv);
   }
 
   // launch geometry for complex lines of length n (lines x outer)
@@ -1756,9 +1777,11 @@ class Batch final : public BatchBase {
   bool finalized_ = false;
 };
 
-// Timed trials of the lines-per-workgroup choices {default, /2, /4, /8} for
-// every pass class of one simulation; the fastest per class goes into the
-// tuning table.  Each trial builds and runs its own copy of the simulation.
+// Timed trials of the lines-per-workgroup choices {default, /2, /4, /8, x2}
+// for every pass class of one simulation, then, where a line length has
+// alternative plan families, each family at the default and half the lines
+// per workgroup; the fastest per class goes into the tuning table.  Each
+// trial builds and runs its own copy of the simulation.
 template <typename T>
 void autotune(int dev, const kw_sim_desc& d0, int steps) {
   kw_sim_desc d = d0;
@@ -1766,28 +1789,40 @@ void autotune(int dev, const kw_sim_desc& d0, int steps) {
   const long long warm = 2;
   d.Nt = warm + steps;
   double best_ms[TC_COUNT];
-  int best_lp[TC_COUNT], best_n[TC_COUNT] = {0};
-  for (int c = 0; c < TC_COUNT; ++c) best_ms[c] = INFINITY;
+  int best_lp[TC_COUNT], best_n[TC_COUNT] = {0}, best_pv[TC_COUNT];
+  for (int c = 0; c < TC_COUNT; ++c) {
+    best_ms[c] = INFINITY;
+    best_pv[c] = -1;
+  }
+  struct Restore {
+    ~Restore() {
+      g_tune_shift = kNoTrial;
+      g_tune_plan = kNoTrial;
+    }
+  };
+  // one trial: builds the simulation with the choice in force, times it
+  auto run = [&](int shift, int plan, double* ms, int* lp, int* nn, int* pv) {
+    std::unique_ptr<Sim<T>> s;
+    {
+      Restore r;
+      g_tune_shift = shift;
+      g_tune_plan = plan;
+      s.reset(new Sim<T>(dev, d, false));
+    }
+    s->trial(warm, steps, ms, lp, nn, pv);
+  };
   // trials: the default, then halving (while any class still changes), then doubling
   int prev_lp[TC_COUNT] = {0}, dflt_lp[TC_COUNT] = {0};
-  bool shrinking = true;
+  bool shrinking = true, has_alt = false;
   for (int shift : {0, 1, 2, 3, -1}) {
     if (shift > 1 && !shrinking) continue;
-    std::unique_ptr<Sim<T>> s;
-    g_tune_shift = shift;
-    try {
-      s.reset(new Sim<T>(dev, d, false));
-    } catch (...) {
-      g_tune_shift = kNoTrial;
-      throw;
-    }
-    g_tune_shift = kNoTrial;
     double ms[TC_COUNT];
-    int lp[TC_COUNT], nn[TC_COUNT];
-    s->trial(warm, steps, ms, lp, nn);
+    int lp[TC_COUNT], nn[TC_COUNT], pv[TC_COUNT];
+    run(shift, kNoTrial, ms, lp, nn, pv);
     bool changed = false;
     for (int c = 0; c < TC_COUNT; ++c) {
       if (nn[c] == 0) continue;
+      if (plan_alternative(nn[c], 0)) has_alt = true;
       const int ref = shift == -1 ? dflt_lp[c] : prev_lp[c];
       if (lp[c] != ref) changed = true;
       prev_lp[c] = lp[c];
@@ -1800,9 +1835,30 @@ void autotune(int dev, const kw_sim_desc& d0, int steps) {
     }
     if (shift > 0 && !changed) shrinking = false;   // every class already at one line per workgroup
   }
+  // plan families 0 (strided), 1 (contiguous), 2 (complex-to-real) on every pass
+  // whose line length has them; a class keeps the default unless another
+  // family is faster (a trial reports -1 for a pass left in its default)
+  for (int plan = 0; has_alt && plan <= 2; ++plan)
+    for (int shift : {0, 1}) {
+      double ms[TC_COUNT];
+      int lp[TC_COUNT], nn[TC_COUNT], pv[TC_COUNT];
+      run(shift, plan, ms, lp, nn, pv);
+      for (int c = 0; c < TC_COUNT; ++c)
+        if (nn[c] > 0 && pv[c] >= 0 && ms[c] < best_ms[c]) {
+          best_ms[c] = ms[c];
+          best_lp[c] = lp[c];
+          best_n[c] = nn[c];
+          best_pv[c] = pv[c];
+        }
+    }
   std::lock_guard<std::mutex> lk(g_tune_mu);
   for (int c = 0; c < TC_COUNT; ++c)
-    if (best_n[c] > 0) g_tune[{(int)sizeof(T), c, best_n[c]}] = best_lp[c];
+    if (best_n[c] > 0) {
+      const std::array<int, 3> key{(int)sizeof(T), c, best_n[c]};
+      g_tune[key] = best_lp[c];
+      if (best_pv[c] >= 0) g_tune_pv[key] = best_pv[c];
+      else g_tune_pv.erase(key);
+    }
 }
 
 // Batch form of the autotuner: the whole batch is built and timed for each
@@ -2186,7 +2242,12 @@ int32_t kw_tune_export(char* buf, int64_t cap, int64_t* len) {
   std::ostringstream os;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    for (auto& kv : g_tune) os << kv.first[0] << ' ' << kv.first[1] << ' ' << kv.first[2] << ' ' << kv.second << '\n';
+    for (auto& kv : g_tune) {
+      os << kv.first[0] << ' ' << kv.first[1] << ' ' << kv.first[2] << ' ' << kv.second;
+      auto it = g_tune_pv.find(kv.first);
+      if (it != g_tune_pv.end()) os << ' ' << it->second;   // plan family, where not the default
+      os << '\n';
+    }
   }
   const std::string s = os.str();
   *len = (int64_t)s.size();
@@ -2202,17 +2263,34 @@ int32_t kw_tune_export(char* buf, int64_t cap, int64_t* len) {
 int32_t kw_tune_import(const char* text) {
   KW_GUARD_BEGIN
   if (!text) raise(KW_ERR_INVALID, "null argument");
+  // one entry per line: precision bytes, class, line length, lines per
+  // workgroup, and optionally the plan family (one the line length has)
   std::istringstream is(text);
-  std::map<std::array<int, 3>, int> add;
-  int p, c, n, l;
-  while (is >> p >> c >> n >> l) {
-    if ((p != 4 && p != 8) || c < 0 || c >= TC_COUNT || !size_supported(n) || l < 1 || l > 256)
+  std::map<std::array<int, 3>, int> add, add_pv;
+  std::string line;
+  while (std::getline(is, line)) {
+    if (line.find_first_not_of(" \t\r") == std::string::npos) continue;   // blank line
+    std::istringstream ls(line);
+    long long v[6];
+    int k = 0;
+    while (k < 6 && ls >> v[k]) ++k;
+    if (!(ls >> std::ws).eof() || (k != 4 && k != 5)) raise(KW_ERR_INVALID, "malformed tuning text");
+    const long long p = v[0], c = v[1], n = v[2], l = v[3];
+    if ((p != 4 && p != 8) || c < 0 || c >= TC_COUNT || n < 1 || n > (1 << 20) || !size_supported((int)n) || l < 1 ||
+        l > 256)
       raise(KW_ERR_INVALID, "malformed tuning entry");
-    add[{p, c, n}] = l;
+    if (k == 5 && !plan_alternative((int)n, (int)std::max(-1LL, std::min(v[4], 1000LL))))
+      raise(KW_ERR_INVALID, "plan family " + std::to_string(v[4]) + " is not compiled for line length " +
+                                std::to_string(n));
+    const std::array<int, 3> key{(int)p, (int)c, (int)n};
+    add[key] = (int)l;
+    add_pv[key] = k == 5 ? (int)v[4] : -1;
   }
-  if (!is.eof()) raise(KW_ERR_INVALID, "malformed tuning text");
   std::lock_guard<std::mutex> lk(g_tune_mu);
   for (auto& kv : add) g_tune[kv.first] = kv.second;
+  for (auto& kv : add_pv)
+    if (kv.second >= 0) g_tune_pv[kv.first] = kv.second;
+    else g_tune_pv.erase(kv.first);
   return KW_OK;
   KW_GUARD_END
 }
@@ -2221,6 +2299,7 @@ int32_t kw_tune_clear(void) {
   KW_GUARD_BEGIN
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune.clear();
+  g_tune_pv.clear();
   return KW_OK;
   KW_GUARD_END
 }

@@ -239,6 +239,7 @@ struct PassCfg {
   int n = 0;        // line length
   int lpw = 1;      // lines (pairs) per workgroup
   int req = 1;      // lines per workgroup asked for (before shrinking to small grids)
+  int pv = -1;      // plan family (plan_alternative) or -1 for the pass's default
   dim3 grid, block;
   size_t shmem = 0;
 };

@@ -399,16 +399,25 @@ __device__ __forceinline__ void inject(const SrcList& s, int t_index, Cx<T> (&v)
 // passes on 16-element plans (128 ... 4096 points) run at two waves per SIMD
 // either way, and there the merged addresses are faster (512-point strided
 // pass of the 512^3 step: 0.94 ms merged, 1.00 ms separate).
-template <typename T, int N, bool CONTIG>
+template <typename T, int N, bool CONTIG, int V = (CONTIG ? 1 : 0)>
 constexpr bool dir_opaque() {
-  return CONTIG || Plan<N, 0>::E != 16;
+  return CONTIG || Plan<N, V>::E != 16;
 }
 
-template <typename T, int N, bool CONTIG, bool TWO = false, bool BS = false>
+// Plan families a pass can be launched with besides its default (the
+// autotuner's choice of radix decomposition and threads per line): at 256
+// points 16 x 16 (family 0), 64 x 4 (1) and 32 x 8 (2); at 512 points 32 x 16
+// (0) and 64 x 8 (1, 2).  PV = -1 in a kernel's template arguments selects the
+// pass's default family.
+constexpr bool plan_alternative(int n, int v) {
+  return (n == 256 || n == 512) && v >= 0 && v <= 2;
+}
+
+template <typename T, int N, bool CONTIG, bool TWO = false, bool BS = false, int PV = -1>
 __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int t_index) {
-  constexpr int V = CONTIG ? (BS ? 4 : 1) : 0;  // plan variant of contiguous x-line passes
+  constexpr int V = PV >= 0 ? PV : (CONTIG ? (BS ? 4 : 1) : 0);   // plan family
   constexpr bool WL = CONTIG && wave_local_lines<N, V>();   // lines inside one wavefront
-  constexpr bool OPQ = dir_opaque<T, N, CONTIG>();
+  constexpr bool OPQ = dir_opaque<T, N, CONTIG, V>();
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -644,9 +653,9 @@ __device__ __forceinline__ void dir_pass_body(const PassArgs<T>& a, int wg, int 
   }
 }
 
-template <typename T, int N, bool CONTIG, bool TWO = false, bool BS = false>
+template <typename T, int N, bool CONTIG, bool TWO = false, bool BS = false, int PV = -1>
 __global__ void __launch_bounds__(512) k_dir_pass(PassArgs<T> a) {
-  dir_pass_body<T, N, CONTIG, TWO, BS>(a, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), a.t);
+  dir_pass_body<T, N, CONTIG, TWO, BS, PV>(a, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), a.t);
 }
 
 // Batched form: one launch covers the workgroups of many simulations of the
@@ -692,10 +701,11 @@ struct SpecArgs {
 // 4 fwd*(M1 S1 + M2 S2)*inv (:stokes), 3 fwd*W*inv, 5 fwd then S1 = inv(M1 .),
 // S2 = inv(M2 .) (the adjoint's K and dK/dc_ref applied to one spectrum).  Specialising on the mode
 // keeps the plain transforms at a small register footprint.
-template <typename T, int N, int MODE, bool BS = false>
+template <typename T, int N, int MODE, bool BS = false, int PV = -1>
 __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
-  constexpr int TPL = Plan<N>::TPL;
-  constexpr int E = Plan<N>::E;
+  constexpr int V = PV >= 0 ? PV : 0;   // plan family
+  constexpr int TPL = Plan<N, V>::TPL;
+  constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   constexpr bool OPQ = MODE != 1;   // every mode but the plain inverse has two transforms
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -707,7 +717,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
   const int kx = xb * lpw + c;
   const bool valid = kx < a.NH;
   Cx<T>* lds = lds_all + c * P;
-  const Twiddles<T, N> tw = load_twiddles<T, N>(a.tw, t);
+  const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(a.tw, t);
   const long long wgb = (long long)(xb * lpw) * a.Sc + a.So * o;   // uniform
   const unsigned lo = (unsigned)(c * a.Sc), sa = (unsigned)a.Sa;
   auto off = [&](int q) { return lo + sa * (unsigned)(t + TPL * q); };
@@ -719,17 +729,17 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
 #pragma unroll
   for (int q = 0; q < E; ++q) v[q] = (valid && ok(q)) ? at32(S1, off(q)) : Cx<T>{0, 0};
   if constexpr (MODE == 1) {
-    line_dft<T, N, true, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
+    line_dft<T, N, true, V, false, BS, OPQ>(v, lds, tw, t, a.ch);
   } else if constexpr (MODE == 3) {
-    line_dft<T, N, false, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
+    line_dft<T, N, false, V, false, BS, OPQ>(v, lds, tw, t, a.ch);
 #pragma unroll
     for (int q = 0; q < E; ++q) v[q] = cscale(v[q], ok(q) ? a.W[t + TPL * q] : (T)0);
-    line_dft<T, N, true, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
+    line_dft<T, N, true, V, false, BS, OPQ>(v, lds, tw, t, a.ch);
   } else if constexpr (MODE == 0 || MODE == 6) {
     // 6: forward of S1 and, where the simulation has one, of S2 (the
     // second-spectrum code would otherwise cost plain forward passes
     // registers, hence occupancy)
-    line_dft<T, N, false, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
+    line_dft<T, N, false, V, false, BS, OPQ>(v, lds, tw, t, a.ch);
     if (MODE == 6 && a.S2) {
       Cx<T>* __restrict__ S2 = a.S2 + wgb;
       if (valid) {
@@ -739,7 +749,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
       }
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = (valid && ok(q)) ? at32(S2, off(q)) : Cx<T>{0, 0};
-      line_dft<T, N, false, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
+      line_dft<T, N, false, V, false, BS, OPQ>(v, lds, tw, t, a.ch);
       if (valid) {
 #pragma unroll
         for (int q = 0; q < E; ++q)
@@ -748,7 +758,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
       return;
     }
   } else if constexpr (MODE == 5) {
-    line_dft<T, N, false, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
+    line_dft<T, N, false, V, false, BS, OPQ>(v, lds, tw, t, a.ch);
     int io = o;
     if (a.axis == 1)
       io = a.Nz > 1 ? (io > a.Nz - io ? a.Nz - io : io) : 0;   // 2-D: outer index = stacked firing
@@ -765,16 +775,16 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
       w[q] = cscale(v[q], m2);
       v[q] = cscale(v[q], m1);
     }
-    line_dft<T, N, true, 0, false, BS, OPQ>(w, lds, tw, t, a.ch);
+    line_dft<T, N, true, V, false, BS, OPQ>(w, lds, tw, t, a.ch);
     if (valid) {
       Cx<T>* __restrict__ S2 = a.S2 + wgb;
 #pragma unroll
       for (int q = 0; q < E; ++q)
         if (ok(q)) at32(S2, off(q)) = w[q];
     }
-    line_dft<T, N, true, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
+    line_dft<T, N, true, V, false, BS, OPQ>(v, lds, tw, t, a.ch);
   } else {
-    line_dft<T, N, false, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
+    line_dft<T, N, false, V, false, BS, OPQ>(v, lds, tw, t, a.ch);
     Cx<T> w[MODE == 4 ? E : 1];
     // In a batch, a MODE 4 launch also covers simulations without a second
     // spectrum (S2 and M2 null): those take w = 0 and never touch S2/M2.
@@ -788,7 +798,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
 #pragma unroll
         for (int q = 0; q < E; ++q) w[q] = Cx<T>{0, 0};
       }
-      line_dft<T, N, false, 0, false, BS, OPQ>(w, lds, tw, t, a.ch);
+      line_dft<T, N, false, V, false, BS, OPQ>(w, lds, tw, t, a.ch);
     }
     // octant index |i| of the outer coordinate (z for y lines, y for z lines)
     int io = o;
@@ -809,7 +819,7 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
       }
       v[q] = r;
     }
-    line_dft<T, N, true, 0, false, BS, OPQ>(v, lds, tw, t, a.ch);
+    line_dft<T, N, true, V, false, BS, OPQ>(v, lds, tw, t, a.ch);
   }
   if (valid) {
 #pragma unroll
@@ -818,9 +828,9 @@ __device__ __forceinline__ void spec_pass_body(const SpecArgs<T>& a, int wg) {
   }
 }
 
-template <typename T, int N, int MODE, bool BS = false>
+template <typename T, int N, int MODE, bool BS = false, int PV = -1>
 __global__ void __launch_bounds__(512) k_spec_pass(SpecArgs<T> a) {
-  spec_pass_body<T, N, MODE, BS>(a, strided_wg<T, N>(blockIdx.x));
+  spec_pass_body<T, N, MODE, BS, PV>(a, strided_wg<T, N>(blockIdx.x));
 }
 
 template <typename T, int N, int MODE>
@@ -852,9 +862,10 @@ struct C2RArgs {
   Chirp<T> ch;
 };
 
-template <typename T, int N, bool BS = false>
+template <typename T, int N, bool BS = false, int PV = -1>
 __device__ __forceinline__ void c2r_body(const C2RArgs<T>& a, int wg) {
-  constexpr int V = BS ? 4 : 2;  // contiguous x lines, complex-to-real plan family (4: chirp-z lines)
+  // contiguous x lines, complex-to-real plan family (4: chirp-z lines)
+  constexpr int V = PV >= 0 ? PV : (BS ? 4 : 2);
   constexpr int TPL = Plan<N, V>::TPL;
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
@@ -911,9 +922,9 @@ __device__ __forceinline__ void c2r_body(const C2RArgs<T>& a, int wg) {
   }
 }
 
-template <typename T, int N, bool BS = false>
+template <typename T, int N, bool BS = false, int PV = -1>
 __global__ void __launch_bounds__(256) k_c2r_x(C2RArgs<T> a) {
-  c2r_body<T, N, BS>(a, blockIdx.x);
+  c2r_body<T, N, BS, PV>(a, blockIdx.x);
 }
 
 template <typename T, int N>
@@ -1332,10 +1343,10 @@ struct AdjArgs {
   int accumulate;   // 0: Pb = contribution, 1: Pb += contribution
 };
 
-template <typename T, int N, bool CONTIG>
+template <typename T, int N, bool CONTIG, int PV = -1>
 __global__ void __launch_bounds__(512) k_adj_pass(AdjArgs<T> a) {
-  constexpr int V = CONTIG ? 1 : 0;
-  constexpr bool OPQ = dir_opaque<T, N, CONTIG>();   // as for the directional passes
+  constexpr int V = PV >= 0 ? PV : (CONTIG ? 1 : 0);   // plan family, as the directional pass's
+  constexpr bool OPQ = dir_opaque<T, N, CONTIG, V>();
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1348,7 +1359,7 @@ __global__ void __launch_bounds__(512) k_adj_pass(AdjArgs<T> a) {
   geo.x_blocks = a.x_blocks;
   int lpw;
   const LineCtx<T, N, V> L =
-      make_ctx<T, N, CONTIG>(geo, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), lpw);
+      make_ctx<T, N, CONTIG, V>(geo, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), lpw);
   Cx<T>* lds = lds_all + L.c * P;
   const Twiddles<T, N, V> tw = load_twiddles<T, N, V>(a.tw, L.t);
 
@@ -1420,9 +1431,9 @@ __global__ void __launch_bounds__(512) k_adj_pass(AdjArgs<T> a) {
 }
 
 // Real lines (pairs) of f -> half spectrum along x (the r2c stage alone).
-template <typename T, int N>
+template <typename T, int N, int PV = -1>
 __global__ void __launch_bounds__(256) k_r2c_x(const T* f, Cx<T>* S, int n_lines, int nhp, const Cx<T>* twt) {
-  constexpr int V = 2;  // contiguous x lines, complex-to-real plan family
+  constexpr int V = PV >= 0 ? PV : 2;  // contiguous x lines: the complex-to-real pass's plan family
   constexpr int E = Plan<N, V>::E;
   constexpr int P = LdsLayout<N>::PITCH;
   constexpr int NH = N / 2 + 1;

@@ -38,11 +38,13 @@ int plan_pitch(int n);
 
 // Return hipSuccess or the launch error; unsupported lengths give hipErrorInvalidValue.
 template <typename T>
-hipError_t launch_dir(int n, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const PassArgs<T>& a);
+// pv: plan family (plan_alternative), -1 for the pass's default.
+hipError_t launch_dir(int n, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const PassArgs<T>& a,
+                      int pv = -1);
 template <typename T>
-hipError_t launch_spec(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<T>& a);
+hipError_t launch_spec(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const SpecArgs<T>& a, int pv = -1);
 template <typename T>
-hipError_t launch_c2r(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const C2RArgs<T>& a);
+hipError_t launch_c2r(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const C2RArgs<T>& a, int pv = -1);
 // Fused y inverse + complex-to-real pass, one workgroup per z plane of nx x ny
 // (plane_c2r_supported: the plane sizes and precisions it is compiled for,
 // given the workgroup LDS limit of the device).
@@ -89,10 +91,11 @@ hipError_t launch_tick(hipStream_t s, int* t_dev);
 
 // Adjoint (gradient w.r.t. c0).
 template <typename T>
-hipError_t launch_adj_pass(int n, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const AdjArgs<T>& a);
+hipError_t launch_adj_pass(int n, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const AdjArgs<T>& a,
+                           int pv = -1);
 template <typename T>
 hipError_t launch_r2c(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const T* f, Cx<T>* S, int n_lines, int nhp,
-                      const Cx<T>* tw);
+                      const Cx<T>* tw, int pv = -1);
 template <typename T>
 hipError_t launch_adj_small(int which, hipStream_t s, const void* const* p, const double* dargs, long long n, int ndim,
                             int nf = 1);

@@ -3,8 +3,40 @@
 
 namespace kw {
 
+template <typename T, int N, int PV>
+static hipError_t adj_alt(bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const AdjArgs<T>& a) {
+  if (contig)
+    hipLaunchKernelGGL((k_adj_pass<T, N, true, PV>), g, b, sh, s, a);
+  else
+    hipLaunchKernelGGL((k_adj_pass<T, N, false, PV>), g, b, sh, s, a);
+  return hipGetLastError();
+}
+template <typename T, int N>
+static hipError_t adj_alt(int pv, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const AdjArgs<T>& a) {
+  switch (pv) {
+    case 0: return adj_alt<T, N, 0>(contig, g, b, sh, s, a);
+    case 1: return adj_alt<T, N, 1>(contig, g, b, sh, s, a);
+    case 2: return adj_alt<T, N, 2>(contig, g, b, sh, s, a);
+  }
+  return hipErrorInvalidValue;
+}
+template <typename T, int N>
+static hipError_t r2c_alt(int pv, dim3 g, dim3 b, size_t sh, hipStream_t s, const T* f, Cx<T>* S, int n_lines,
+                          int nhp, const Cx<T>* tw) {
+  switch (pv) {
+    case 0: hipLaunchKernelGGL((k_r2c_x<T, N, 0>), g, b, sh, s, f, S, n_lines, nhp, tw); break;
+    case 1: hipLaunchKernelGGL((k_r2c_x<T, N, 1>), g, b, sh, s, f, S, n_lines, nhp, tw); break;
+    case 2: hipLaunchKernelGGL((k_r2c_x<T, N, 2>), g, b, sh, s, f, S, n_lines, nhp, tw); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
-hipError_t launch_adj_pass(int n, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const AdjArgs<T>& a) {
+hipError_t launch_adj_pass(int n, bool contig, dim3 g, dim3 b, size_t sh, hipStream_t s, const AdjArgs<T>& a,
+                           int pv) {
+  if (plan_alternative(n, pv))
+    return n == 256 ? adj_alt<T, 256>(pv, contig, g, b, sh, s, a) : adj_alt<T, 512>(pv, contig, g, b, sh, s, a);
   switch (n) {
 #define X(n_)                                                             \
   case n_:                                                                \
@@ -21,7 +53,10 @@ hipError_t launch_adj_pass(int n, bool contig, dim3 g, dim3 b, size_t sh, hipStr
 
 template <typename T>
 hipError_t launch_r2c(int n, dim3 g, dim3 b, size_t sh, hipStream_t s, const T* f, Cx<T>* S, int n_lines, int nhp,
-                      const Cx<T>* tw) {
+                      const Cx<T>* tw, int pv) {
+  if (plan_alternative(n, pv))
+    return n == 256 ? r2c_alt<T, 256>(pv, g, b, sh, s, f, S, n_lines, nhp, tw)
+                    : r2c_alt<T, 512>(pv, g, b, sh, s, f, S, n_lines, nhp, tw);
   switch (n) {
 #define X(n_)                                                                      \
   case n_:                                                                         \
@@ -70,8 +105,9 @@ hipError_t launch_adj_small(int which, hipStream_t s, const void* const* p, cons
 }
 
 #define KW_INST(T)                                                                                                  \
-  template hipError_t launch_adj_pass<T>(int, bool, dim3, dim3, size_t, hipStream_t, const AdjArgs<T>&);           \
-  template hipError_t launch_r2c<T>(int, dim3, dim3, size_t, hipStream_t, const T*, Cx<T>*, int, int, const Cx<T>*); \
+  template hipError_t launch_adj_pass<T>(int, bool, dim3, dim3, size_t, hipStream_t, const AdjArgs<T>&, int);      \
+  template hipError_t launch_r2c<T>(int, dim3, dim3, size_t, hipStream_t, const T*, Cx<T>*, int, int, const Cx<T>*,  \
+                                    int);                                                                           \
   template hipError_t launch_adj_small<T>(int, hipStream_t, const void* const*, const double*, long long, int, int);
 KW_INST(float)
 KW_INST(double)

@@ -2,11 +2,15 @@
 
 The reference tunes its transforms with FFTW.MEASURE when it creates its
 plans (fft_utils.jl:44-62).  Here each pass is a hand-written line-FFT
-kernel whose one free launch parameter is the number of lines a workgroup
-carries; ``autotune`` times the candidates on the device for a given
-simulation and keeps the fastest per (precision, pass, line length) in the
-library's process-wide table, which every later simulation or batch reads.
-The table can be saved to and loaded from a small text cache file.
+kernel with two free choices: the number of lines a workgroup carries and,
+for line lengths 256 and 512, the plan family (the radix schedule and
+threads per line: 0 = the strided passes' plan, 1 = the contiguous x pass's,
+2 = the complex-to-real pass's).  ``autotune`` times the candidates on the
+device for a given simulation and keeps the fastest per (precision, pass,
+line length) in the library's process-wide table, which every later single
+simulation reads (batches take the lines per workgroup and the default plan
+families).  The table can be saved to and loaded from a small text cache
+file: one entry per line, ``precision class length lines [plan]``.
 """
 from __future__ import annotations
 
@@ -72,13 +76,21 @@ def table_text() -> str:
     return buf.value.decode()
 
 
+def _entries():
+    for line in table_text().splitlines():
+        v = [int(x) for x in line.split()]
+        yield (v[0], PASS_NAMES[v[1]], v[2]), v[3], (v[4] if len(v) == 5 else None)
+
+
 def table() -> Dict[Tuple[int, str, int], int]:
     """{(precision bytes, pass name, line length): lines per workgroup}."""
-    out = {}
-    for line in table_text().splitlines():
-        p, c, n, l = (int(x) for x in line.split())
-        out[(p, PASS_NAMES[c], n)] = l
-    return out
+    return {k: l for k, l, _ in _entries()}
+
+
+def plans() -> Dict[Tuple[int, str, int], int]:
+    """{(precision bytes, pass name, line length): plan family} for the entries
+    that name one (the others use the pass's default family)."""
+    return {k: v for k, _, v in _entries() if v is not None}
 
 
 def load_table(path: str) -> None:

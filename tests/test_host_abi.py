@@ -152,9 +152,24 @@ def test_tuning_table_text_round_trip():
         with pytest.raises(_lib.KWaveHIPError):
             _lib.check(lib.kw_tune_import(b"4 0 256 x\n"))
         assert len(tuning.table()) == 2                          # failed imports add nothing
+        assert tuning.plans() == {}
+        # optional fifth column: plan family, only where the line length has one
+        _lib.check(lib.kw_tune_import(b"4 5 512 4 0\n\n8 1 256 16 1\n"))
+        assert tuning.plans() == {(4, "c2r_x", 512): 0, (8, "dir_y", 256): 1}
+        assert tuning.table()[(4, "c2r_x", 512)] == 4
+        assert "4 5 512 4 0" in tuning.table_text().splitlines()
+        for bad in (b"4 0 96 8 1\n",        # no alternative plans at 96
+                    b"4 0 256 8 3\n",       # no plan family 3
+                    b"4 0 256 8 1 7\n",     # six columns
+                    b"4 0 256\n"):          # three columns
+            with pytest.raises(_lib.KWaveHIPError):
+                _lib.check(lib.kw_tune_import(bad))
+        assert len(tuning.plans()) == 2
+        _lib.check(lib.kw_tune_import(b"4 5 512 4\n"))          # re-import without a plan: the default again
+        assert tuning.plans() == {(8, "dir_y", 256): 1}
     finally:
         tuning.clear_table()
-    assert tuning.table() == {}
+    assert tuning.table() == {} and tuning.plans() == {}
 
 
 def test_errors_from_every_translation_unit_reach_kw_last_error():
