@@ -133,10 +133,10 @@ def cpu_baseline(n_steps_max: int = 20, budget_s: float = 20.0):
             "cpu_model": model, "cpus_visible": visible, "cpus_in_host": total}
 
 
-def bench_batch(a):
+def bench_batch(a, tune: bool = True, seq_steps_max: int = 200):
     """64 uneven 2-D firings on 256^2..1024^2 grids; batched (one launch per pass per line
     length, line lengths on concurrent streams, hipGraph step) against the same simulations
-    run one after another."""
+    run one after another.  With tune=False the batch runs once, with the default shapes."""
     import numpy as np
     import kwave_amd as K
     from kwave_amd.configs import uneven_2d_batch
@@ -163,17 +163,23 @@ def bench_batch(a):
         return u2 - u1
 
     K.tuning.clear_table()
+    print(f"[bench] batch of {len(sims)}: {a.steps} steps", file=sys.stderr, flush=True)
     default_secs = run_batch()
-    # autotuner: trials of the whole batch (untimed)
-    tt = time.perf_counter()
-    K.tuning.autotune_batch(sims, steps=10, data_cast=np.float32, **kw)
-    tune_s = time.perf_counter() - tt
-    table = K.tuning.table()
-    t1 = 0.0
-    t2 = run_batch()
+    tune_s, table = None, {}
+    t1, t2 = 0.0, default_secs
+    if tune:
+        # autotuner: trials of the whole batch (untimed)
+        tt = time.perf_counter()
+        K.tuning.autotune_batch(sims, steps=10, data_cast=np.float32, **kw)
+        tune_s = time.perf_counter() - tt
+        table = K.tuning.table()
+        t2 = run_batch()
+        K.tuning.clear_table()
     batched = vox * a.steps / (t2 - t1)
     # the same simulations, one plain launch sequence each (truncated run; a rate)
-    seq_steps = max(1, min(a.steps, 200))
+    seq_steps = max(1, min(a.steps, seq_steps_max))
+    print(f"[bench] the same {len(sims)} simulations one after another: {seq_steps} steps each",
+          file=sys.stderr, flush=True)
     tot = 0.0
     for d, k in zip(descs, keep):
         s = K.Simulation(d, k)
@@ -198,10 +204,10 @@ def bench_batch(a):
         "tuning_table": {f"{k[1]}:{k[2]}": v for k, v in sorted(table.items())},
         "sequential_sample": f"{seq_steps} steps per simulation after {a.warmup} warm-up steps",
     }
-    print(json.dumps(out))
+    return out
 
 
-def bench_adjoint(a):
+def bench_adjoint(a, nseq_max: int = 8):
     """Adjoint form: 2-D 512^2 ring-array misfit gradient, FP32, checkpoints every 50 steps.
 
     Observed data come from a forward run on the true model (untimed); the timed
@@ -231,7 +237,7 @@ def bench_adjoint(a):
     vox = int(np.prod(kg.grid_size()))
     secs = t2 - t1
     # the same gradient one firing after another (stack=1) on a subset, for the batched/sequential ratio
-    nseq = min(8, len(firings))
+    nseq = min(nseq_max, len(firings))
     print(f"[bench] sequential gradient over {nseq} firings", file=sys.stderr, flush=True)
     t3 = time.perf_counter()
     _, g_seq = K.gradient_c0(kg, med0, srcs[:nseq], sens[:nseq], obs[:nseq], checkpoint_every=50,
@@ -277,7 +283,7 @@ def bench_adjoint(a):
         "stacked_over_sequential_speedup": seq_per_firing / (secs / len(firings)),
         "loss": loss_total, "fd_check": fd,
     }
-    print(json.dumps(out))
+    return out
 
 
 def bench_c4(a):
@@ -520,6 +526,8 @@ def main():
     ap.add_argument("--fd-pixels", type=int, default=32)
     ap.add_argument("--c4-maps", choices=["fp16", "fp32"], default="fp16")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the short batched and adjoint runs reported under 'forms'")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU slab decomposition "
@@ -531,9 +539,11 @@ def main():
     if a.config == "batch":
         if a.steps == 1200:
             a.steps = 1500
-        return bench_batch(a)
+        print(json.dumps(bench_batch(a)))
+        return
     if a.config == "adjoint":
-        return bench_adjoint(a)
+        print(json.dumps(bench_adjoint(a)))
+        return
     if a.config == "c4":
         if a.steps == 1200:
             a.steps = 200
@@ -574,6 +584,7 @@ def main():
     fp64_check = None
     if not a.no_fp64:
         fp64_check = accuracy_check(a.N, a.accuracy_nt)
+    forms = None if a.no_forms else other_forms()
 
     N = a.N
     vox = N ** 3
@@ -627,6 +638,7 @@ def main():
         "kernels": kernels,
         "fp32_vs_fp64_rel_l2": fp64_check,
         "setup_s": setup_s,
+        "forms": forms,
     }
     if not a.no_cpu:
         print("[bench] cpu baseline (oracle) ...", file=sys.stderr, flush=True)
@@ -634,6 +646,24 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+def other_forms():
+    """Short runs of the batched and adjoint forms beside the headline step, so that one
+    default run records their batched-over-sequential ratios: the 64-simulation uneven
+    2-D batch (300 steps, default launch shapes; 50 steps per simulation one after another)
+    and the 128-firing 512^2 ring gradient stacked (2 firings one after another)."""
+    b = bench_batch(argparse.Namespace(warmup=20, steps=300), tune=False, seq_steps_max=50)
+    d = bench_adjoint(argparse.Namespace(firings=128, fd_pixels=0), nseq_max=2)
+    return {
+        "batch_64_uneven_2d": {k: b[k] for k in ("value", "unit", "steps", "ms_per_step", "sequential_value",
+                                                 "batched_over_sequential", "sequential_sample")},
+        "adjoint_ring512_128_firings": {"seconds": d["seconds_per_firing"] * 128,
+                                        "seconds_per_firing": d["seconds_per_firing"],
+                                        "sequential_seconds_per_firing": d["sequential_seconds_per_firing"],
+                                        "stacked_over_sequential_speedup": d["stacked_over_sequential_speedup"],
+                                        "forward_voxel_steps_per_s": d["value"], "sequential_sample": "2 firings"},
+    }
 
 
 def accuracy_check(N: int, nt: int):
