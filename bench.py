@@ -149,8 +149,8 @@ def bench_batch(a, tune: bool = True, seq_steps_max: int = 200):
         keep.append(k)
     vox = sum(int(np.prod(s[0].grid_size())) for s in sims)
 
-    def run_batch():
-        b = K.Batch(descs, keep, use_graph=True)
+    def run_batch(persistent=False):
+        b = K.Batch(descs, keep, use_graph=True, persistent=persistent)
         b.run(a.warmup)
         b.synchronize()
         u1 = time.perf_counter()
@@ -165,7 +165,7 @@ def bench_batch(a, tune: bool = True, seq_steps_max: int = 200):
     K.tuning.clear_table()
     print(f"[bench] batch of {len(sims)}: {a.steps} steps", file=sys.stderr, flush=True)
     default_secs = run_batch()
-    tune_s, table = None, {}
+    tune_s, table, persistent_secs = None, {}, None
     t1, t2 = 0.0, default_secs
     if tune:
         # autotuner: trials of the whole batch (untimed)
@@ -174,6 +174,7 @@ def bench_batch(a, tune: bool = True, seq_steps_max: int = 200):
         tune_s = time.perf_counter() - tt
         table = K.tuning.table()
         t2 = run_batch()
+        persistent_secs = run_batch(persistent=True)
         K.tuning.clear_table()
     batched = vox * a.steps / (t2 - t1)
     # the same simulations, one plain launch sequence each (truncated run; a rate)
@@ -201,6 +202,7 @@ def bench_batch(a, tune: bool = True, seq_steps_max: int = 200):
                    "voxels_per_step": vox, "parallelism": "single GPU"},
         "sequential_value": sequential, "batched_over_sequential": batched / sequential,
         "untuned_batched_value": vox * a.steps / default_secs, "autotune_s": tune_s,
+        "persistent_batched_value": vox * a.steps / persistent_secs if persistent_secs else None,
         "tuning_table": {f"{k[1]}:{k[2]}": v for k, v in sorted(table.items())},
         "sequential_sample": f"{seq_steps} steps per simulation after {a.warmup} warm-up steps",
     }

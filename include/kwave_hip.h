@@ -284,10 +284,19 @@ int32_t kw_sim_destroy(kw_sim* sim);
  * Batches of independent simulations (no reference counterpart: the
  * reference runs one simulation per call).  Every pass is one launch per
  * line length over all simulations, the step index lives on the device and,
- * with use_graph, one step is captured in a hipGraph and replayed.  All
- * simulations share precision, dimensionality and Nt; no time reversal.
- * Sensor data and fields are read per simulation like kw_sim_get_*.
+ * with KW_BATCH_GRAPH in use_graph (a flags word; 0 and 1 keep their old
+ * meaning), one step is captured in a hipGraph and replayed.  With
+ * KW_BATCH_PERSISTENT each batched pass runs as a persistent grid (as many
+ * workgroups as fit the device at once, each looping over the pass's
+ * workgroup tiles) instead of one workgroup per tile; KW_BATCH_GRID8(n)
+ * (n >= 1, with KW_BATCH_PERSISTENT) caps that grid at 8 n workgroups, e.g.
+ * to leave room on the device for other work.  Other bits: KW_ERR_INVALID.  All simulations share precision, dimensionality and Nt;
+ * no time reversal.  Sensor data and fields are read per simulation like
+ * kw_sim_get_*.
  */
+#define KW_BATCH_GRAPH 1
+#define KW_BATCH_PERSISTENT 2
+#define KW_BATCH_GRID8(n) ((int32_t)(n) << 8)
 typedef struct kw_batch kw_batch;
 int32_t kw_batch_create(kw_ctx* ctx, const kw_sim_desc* descs, int32_t n_sims, int32_t use_graph, kw_batch** out);
 int32_t kw_batch_run(kw_batch* batch, int64_t n_steps);

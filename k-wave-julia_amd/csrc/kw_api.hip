@@ -1420,8 +1420,13 @@ struct BatchBase {
 template <typename T>
 class Batch final : public BatchBase {
  public:
-  Batch(int device, const kw_sim_desc* descs, int n, bool use_graph) : dev_(device), use_graph_(use_graph) {
+  Batch(int device, const kw_sim_desc* descs, int n, bool use_graph, bool persistent = false, int grid_cap = 0)
+      : dev_(device), use_graph_(use_graph), cap_(grid_cap) {
     HIPC(hipSetDevice(dev_));
+    if (persistent) {
+      HIPC(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev_));
+      cus_ = std::max(cus_, 1);
+    }
     if (n < 1) raise(KW_ERR_INVALID, "empty batch");
     for (int i = 0; i < n; ++i) {
       if (descs[i].ndim != descs[0].ndim) raise(KW_ERR_INVALID, "a batch needs one dimensionality");
@@ -1695,16 +1700,18 @@ class Batch final : public BatchBase {
     switch (g.kind) {
       case DIR_Y: case DIR_Z: case DIR_X:
         HIPC(launch_dir_batch<T>(g.n, g.kind == DIR_X, g.kind == DIR_X && g.two, g.nwg, g.block, g.shmem, st, reinterpret_cast<PassArgs<T>*>(g.args.p),
-                                 reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p), reinterpret_cast<int*>(t_dev_.p)));
+                                 reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p), reinterpret_cast<int*>(t_dev_.p),
+                                 cus_, cap_));
         break;
       case C2R:
         HIPC(launch_c2r_batch<T>(g.n, g.nwg, g.block, g.shmem, st, reinterpret_cast<C2RArgs<T>*>(g.args.p), reinterpret_cast<int*>(g.wg_sim.p),
-                                 reinterpret_cast<int*>(g.wg_first.p)));
+                                 reinterpret_cast<int*>(g.wg_first.p), cus_, cap_));
         break;
       default: {
         const int mode = g.kind == SPEC_Y_FWD ? 0 : (g.kind == SPEC_Y_INV ? 1 : 2);
         HIPC(launch_spec_batch<T>(g.n, mode, g.two && mode != 1, g.nwg, g.block, g.shmem, st,
-                                  reinterpret_cast<SpecArgs<T>*>(g.args.p), reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p)));
+                                  reinterpret_cast<SpecArgs<T>*>(g.args.p), reinterpret_cast<int*>(g.wg_sim.p), reinterpret_cast<int*>(g.wg_first.p),
+                                  cus_, cap_));
       }
     }
   }
@@ -1763,6 +1770,8 @@ class Batch final : public BatchBase {
 
   int dev_;
   bool use_graph_;
+  int cus_ = 0;   // persistent grids: the device's compute units (0: one workgroup per tile)
+  int cap_ = 0;   // persistent grids: at most this many workgroups (0: as many as fit)
   hipStream_t stream_ = nullptr;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
@@ -2325,11 +2334,15 @@ int32_t kw_batch_create(kw_ctx* ctx, const kw_sim_desc* descs, int32_t n_sims, i
   if (n_sims < 1) raise(KW_ERR_INVALID, "empty batch");
   for (int i = 1; i < n_sims; ++i)
     if (descs[i].precision != descs[0].precision) raise(KW_ERR_INVALID, "a batch needs one precision");
+  if (use_graph < 0 || (use_graph & 0xFC) != 0) raise(KW_ERR_INVALID, "unknown batch flags");
+  const bool graph = (use_graph & KW_BATCH_GRAPH) != 0, persistent = (use_graph & KW_BATCH_PERSISTENT) != 0;
+  const int cap = 8 * (use_graph >> 8);   // KW_BATCH_GRID8(n): at most 8 n workgroups
+  if (cap > 0 && !persistent) raise(KW_ERR_INVALID, "a grid cap needs KW_BATCH_PERSISTENT");
   std::unique_ptr<kw_batch> b(new kw_batch);
   if (descs[0].precision == 4)
-    b->impl.reset(new Batch<float>(ctx->device, descs, n_sims, use_graph != 0));
+    b->impl.reset(new Batch<float>(ctx->device, descs, n_sims, graph, persistent, cap));
   else if (descs[0].precision == 8)
-    b->impl.reset(new Batch<double>(ctx->device, descs, n_sims, use_graph != 0));
+    b->impl.reset(new Batch<double>(ctx->device, descs, n_sims, graph, persistent, cap));
   else
     raise(KW_ERR_INVALID, "precision must be 4 or 8");
   *out = b.release();

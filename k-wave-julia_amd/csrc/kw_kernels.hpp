@@ -40,11 +40,15 @@ __device__ __forceinline__ int xcd_wg(int b, int n) {
 // the plain order (decided at compile time: a runtime test cost the 256-point
 // passes about 1 %).
 template <typename T, int N>
-__device__ __forceinline__ int strided_wg(int b) {
+__device__ __forceinline__ int strided_wg(int b, int n) {
   if constexpr ((512 / Plan<N, 0>::TPL) * 2 * (int)sizeof(T) < 128)
-    return xcd_wg(b, (int)gridDim.x);
+    return xcd_wg(b, n);
   else
     return b;
+}
+template <typename T, int N>
+__device__ __forceinline__ int strided_wg(int b) {
+  return strided_wg<T, N>(b, (int)gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -658,17 +662,45 @@ __global__ void __launch_bounds__(512) k_dir_pass(PassArgs<T> a) {
   dir_pass_body<T, N, CONTIG, TWO, BS, PV>(a, CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x), a.t);
 }
 
-// Batched form: one launch covers the workgroups of many simulations of the
-// same line length; wg_sim maps a workgroup to its simulation, wg_first is
-// each simulation's first workgroup, *t_dev the current 1-based step.
-template <typename T, int N, bool CONTIG, bool TWO = false>
-__global__ void __launch_bounds__(512) k_dir_pass_batch(const PassArgs<T>* __restrict__ args,
-                                                        const int* __restrict__ wg_sim,
-                                                        const int* __restrict__ wg_first, const int* t_dev) {
-  const int b = CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x);
+// Batched form: one launch covers the nwg workgroup tiles of many
+// simulations of the same line length; wg_sim maps a tile to its simulation,
+// wg_first is each simulation's first tile, *t_dev the current 1-based step.
+// P = false: one workgroup per tile (grid nwg).  P = true (persistent): a
+// smaller grid, a multiple of 8 workgroups, walks the tiles with stride
+// gridDim.x, so tile w stays on XCD w % 8 as in the one-tile order.  The
+// loop costs registers: values the tile computes from the lane index alone
+// are kept live across iterations (FP32: +5 to +80 VGPRs per kernel, the
+// 1024-point directional passes spill), whether the tile is inlined or
+// called (a called tile is compiled without the kernel's launch bounds and
+// spills more).  Persistent instances exist for the line lengths of large
+// batched groups.
+constexpr bool persistent_length(int n) { return n == 256 || n == 384 || n == 512 || n == 768 || n == 1024; }
+
+template <typename T, int N, bool CONTIG, bool TWO>
+__device__ __forceinline__ void dir_tile(const PassArgs<T>* __restrict__ args, const int* __restrict__ wg_sim,
+                                      const int* __restrict__ wg_first, int t, int w, int nwg) {
+  const int b = CONTIG ? w : strided_wg<T, N>(w, nwg);
   const int s = cmem(wg_sim)[b];
   const PassArgs<T> a = cmem(args)[s];
-  dir_pass_body<T, N, CONTIG, TWO>(a, b - cmem(wg_first)[s], *cmem(t_dev));
+  dir_pass_body<T, N, CONTIG, TWO>(a, b - cmem(wg_first)[s], t);
+}
+
+template <typename T, int N, bool CONTIG, bool TWO = false, bool P = false>
+__global__ void __launch_bounds__(512) k_dir_pass_batch(const PassArgs<T>* __restrict__ args,
+                                                        const int* __restrict__ wg_sim,
+                                                        const int* __restrict__ wg_first, const int* t_dev, int nwg) {
+  if constexpr (P) {
+    const int t = *cmem(t_dev);
+    for (int w = blockIdx.x; w < nwg; w += gridDim.x) {
+      if (w != (int)blockIdx.x) __syncthreads();   // the previous tile is done with the LDS
+      dir_tile<T, N, CONTIG, TWO>(args, wg_sim, wg_first, t, w, nwg);
+    }
+  } else {
+    const int b = CONTIG ? (int)blockIdx.x : strided_wg<T, N>(blockIdx.x);
+    const int s = cmem(wg_sim)[b];
+    const PassArgs<T> a = cmem(args)[s];
+    dir_pass_body<T, N, CONTIG, TWO>(a, b - cmem(wg_first)[s], *cmem(t_dev));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -834,13 +866,29 @@ __global__ void __launch_bounds__(512) k_spec_pass(SpecArgs<T> a) {
 }
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__(512) k_spec_pass_batch(const SpecArgs<T>* __restrict__ args,
-                                                         const int* __restrict__ wg_sim,
-                                                         const int* __restrict__ wg_first) {
-  const int b = strided_wg<T, N>(blockIdx.x);
+__device__ __forceinline__ void spec_tile(const SpecArgs<T>* __restrict__ args, const int* __restrict__ wg_sim,
+                                       const int* __restrict__ wg_first, int w, int nwg) {
+  const int b = strided_wg<T, N>(w, nwg);
   const int s = cmem(wg_sim)[b];
   const SpecArgs<T> a = cmem(args)[s];
   spec_pass_body<T, N, MODE>(a, b - cmem(wg_first)[s]);
+}
+
+template <typename T, int N, int MODE, bool P = false>
+__global__ void __launch_bounds__(512) k_spec_pass_batch(const SpecArgs<T>* __restrict__ args,
+                                                         const int* __restrict__ wg_sim,
+                                                         const int* __restrict__ wg_first, int nwg) {
+  if constexpr (P) {   // tiles as in k_dir_pass_batch
+    for (int w = blockIdx.x; w < nwg; w += gridDim.x) {
+      if (w != (int)blockIdx.x) __syncthreads();
+      spec_tile<T, N, MODE>(args, wg_sim, wg_first, w, nwg);
+    }
+  } else {
+    const int b = strided_wg<T, N>(blockIdx.x);
+    const int s = cmem(wg_sim)[b];
+    const SpecArgs<T> a = cmem(args)[s];
+    spec_pass_body<T, N, MODE>(a, b - cmem(wg_first)[s]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -928,12 +976,27 @@ __global__ void __launch_bounds__(256) k_c2r_x(C2RArgs<T> a) {
 }
 
 template <typename T, int N>
+__device__ __forceinline__ void c2r_tile(const C2RArgs<T>* __restrict__ args, const int* __restrict__ wg_sim,
+                                      const int* __restrict__ wg_first, int w) {
+  const int s = cmem(wg_sim)[w];
+  const C2RArgs<T> a = cmem(args)[s];
+  c2r_body<T, N>(a, w - cmem(wg_first)[s]);
+}
+
+template <typename T, int N, bool P = false>
 __global__ void __launch_bounds__(256) k_c2r_x_batch(const C2RArgs<T>* __restrict__ args,
                                                      const int* __restrict__ wg_sim,
-                                                     const int* __restrict__ wg_first) {
-  const int s = cmem(wg_sim)[blockIdx.x];
-  const C2RArgs<T> a = cmem(args)[s];
-  c2r_body<T, N>(a, blockIdx.x - cmem(wg_first)[s]);
+                                                     const int* __restrict__ wg_first, int nwg) {
+  if constexpr (P) {   // tiles as in k_dir_pass_batch
+    for (int w = blockIdx.x; w < nwg; w += gridDim.x) {
+      if (w != (int)blockIdx.x) __syncthreads();
+      c2r_tile<T, N>(args, wg_sim, wg_first, w);
+    }
+  } else {
+    const int s = cmem(wg_sim)[blockIdx.x];
+    const C2RArgs<T> a = cmem(args)[s];
+    c2r_body<T, N>(a, blockIdx.x - cmem(wg_first)[s]);
+  }
 }
 
 // ---------------------------------------------------------------------------

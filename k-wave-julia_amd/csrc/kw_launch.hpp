@@ -74,16 +74,39 @@ hipError_t launch_smooth_helpers(int which, unsigned nb, hipStream_t s, const T*
 template <typename T>
 hipError_t launch_filter(hipStream_t s, const T* p_ts, const FiltArgs& a);
 
-// Batched forms (one launch over the workgroups of many simulations).
+// Batched forms (one launch over the nwg workgroup tiles of many simulations).
+// cus = 0: one workgroup per tile; cus > 0 (the device's compute units): a
+// persistent grid of as many workgroups as fit the device at once, at most
+// cap (0: no cap), rounded down to a multiple of 8, at most nwg, each
+// walking the tiles.
 template <typename T>
 hipError_t launch_dir_batch(int n, bool contig, bool two, unsigned nwg, dim3 b, size_t sh, hipStream_t s, const PassArgs<T>* args,
-                            const int* wg_sim, const int* wg_first, const int* t_dev);
+                            const int* wg_sim, const int* wg_first, const int* t_dev, int cus = 0, int cap = 0);
 template <typename T>
 hipError_t launch_spec_batch(int n, int mode, bool two, unsigned nwg, dim3 b, size_t sh, hipStream_t s,
-                             const SpecArgs<T>* args, const int* wg_sim, const int* wg_first);
+                             const SpecArgs<T>* args, const int* wg_sim, const int* wg_first, int cus = 0, int cap = 0);
 template <typename T>
 hipError_t launch_c2r_batch(int n, unsigned nwg, dim3 b, size_t sh, hipStream_t s, const C2RArgs<T>* args,
-                            const int* wg_sim, const int* wg_first);
+                            const int* wg_sim, const int* wg_first, int cus = 0, int cap = 0);
+
+// Launch of one batched kernel over nwg tiles (grid as described above).
+template <typename... P, typename... A>
+hipError_t launch_tiles(void (*kern)(P...), unsigned nwg, dim3 b, size_t sh, hipStream_t s, int cus, int cap,
+                        A... args) {
+  unsigned g = nwg;
+  if (cus > 0) {
+    int per_cu = 0;
+    const hipError_t e =
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), (int)b.x, sh);
+    if (e != hipSuccess) return e;
+    unsigned fit = (unsigned)(per_cu > 0 ? per_cu : 1) * (unsigned)cus;
+    if (cap > 0 && (unsigned)cap < fit) fit = (unsigned)cap;
+    fit = fit / 8u * 8u;
+    if (fit >= 8u && fit < nwg) g = fit;
+  }
+  hipLaunchKernelGGL(kern, dim3(g), b, sh, s, args..., (int)nwg);
+  return hipGetLastError();
+}
 template <typename T>
 hipError_t launch_record_batch(hipStream_t s, const RecArgs<T>* args, const int* sens_sim, const int* sens_first,
                                int n_total, const int* t_dev);

@@ -5,6 +5,8 @@ The reference runs one simulation per call; a sweep over transmit elements
 takes that list and runs it through ``kw_batch_*``: each pass of the time
 step is one launch per line length over all simulations, the step index
 lives on the device, and one step is captured in a hipGraph and replayed.
+With ``persistent`` every batched pass runs as a persistent grid (as many
+workgroups as fit the device at once, each looping over the pass's tiles).
 Results are the same ``SimulationOutput`` objects the single call returns.
 """
 from __future__ import annotations
@@ -22,12 +24,17 @@ from .solver import _DP, build_desc
 class Batch:
     """Handle over kw_batch_create .. kw_batch_destroy."""
 
-    def __init__(self, descs: Sequence["_lib.SimDesc"], keep: list, use_graph: bool = True, device: int = 0):
+    GRAPH, PERSISTENT = 1, 2   # kw_batch_create flags (KW_BATCH_GRAPH, KW_BATCH_PERSISTENT)
+
+    def __init__(self, descs: Sequence["_lib.SimDesc"], keep: list, use_graph: bool = True, device: int = 0,
+                 persistent: bool = False, grid8: int = 0):
         self._lib = _lib.load()
         ctx = _lib.context(device)
         arr = (_lib.SimDesc * len(descs))(*descs)
         h = C.c_void_p()
-        _lib.check(self._lib.kw_batch_create(ctx, arr, len(descs), 1 if use_graph else 0, C.byref(h)))
+        # grid8 > 0 (persistent only): at most 8 * grid8 workgroups per batched launch
+        flags = (self.GRAPH if use_graph else 0) | (self.PERSISTENT if persistent else 0) | (int(grid8) << 8)
+        _lib.check(self._lib.kw_batch_create(ctx, arr, len(descs), flags, C.byref(h)))
         self._h = h
         self.descs = list(descs)
         del keep
@@ -73,13 +80,14 @@ class Batch:
 
 def kspace_first_order_batch(sims: Sequence[Tuple], *, data_cast=np.float32, use_graph: bool = True,
                              device: int = 0, autotune: bool = False, tune_cache: str | None = None,
-                             **kwargs) -> List[SimulationOutput]:
+                             persistent: bool = False, **kwargs) -> List[SimulationOutput]:
     """Run a list of (kgrid, medium, source, sensor) in one batch.
 
     Keyword arguments are those of ``kspace_first_order`` and apply to every
     simulation; all simulations must share dimensionality and Nt.  With
     ``autotune`` the launch shapes are first timed on the whole batch
-    (``tune_cache``: text file the table is kept in).
+    (``tune_cache``: text file the table is kept in); ``persistent`` selects
+    the persistent-grid launches.
     """
     if autotune:
         from .tuning import autotune_batch
@@ -93,7 +101,7 @@ def kspace_first_order_batch(sims: Sequence[Tuple], *, data_cast=np.float32, use
         keep.append(k)
         infos.append(info)
     outs = []
-    with Batch(descs, keep, use_graph=use_graph, device=device) as b:
+    with Batch(descs, keep, use_graph=use_graph, device=device, persistent=persistent) as b:
         b.run(infos[0]["Nt"])
         b.synchronize()
         for i, (kgrid, medium, source, sensor) in enumerate(sims):

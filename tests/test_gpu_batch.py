@@ -122,3 +122,45 @@ def test_batch_odd_and_long_lines():
             for k in outs[i].keys():
                 e = rel_l2(outs[i][k], ref[k])
                 assert e <= tol, (i, k, e)
+
+
+@pytest.mark.parametrize("grid8", [1, 0], ids=["grid8", "fill"])
+def test_batch_persistent_grids(grid8):
+    """Persistent-grid launches (each workgroup walking many tiles) keep parity; with an
+    8-workgroup grid every batched pass loops over its tiles."""
+    print(f"\n[case] batch persistent grid8={grid8} 2d f64 + 3d f64", flush=True)
+    from kwave_amd.batch import Batch
+    for names, nt in (([("hetc_absorb_psrc", (64, 64)), ("stokes_p0_psrc", (64, 96)), ("psrc", (128, 48)),
+                        ("usrc_urec", (48, 48))], 30),
+                      ([("hetc_absorb_nonlin_psrc", (32, 24, 16)), ("psrc", (16, 32, 24))], 20)):
+        sims, kw = _sims(names, nt=nt)
+        descs, keep = [], []
+        for s in sims:
+            d, k, _ = K.build_desc(*copy.deepcopy(s), data_cast=np.float64, **kw)
+            descs.append(d)
+            keep.append(k)
+        for use_graph in (True, False):
+            with Batch(descs, keep, use_graph=use_graph, persistent=True, grid8=grid8) as b:
+                b.run(nt)
+                b.synchronize()
+                got = [b.sensor(i, "p") for i in range(len(sims))]
+            for i, (kg, med, src, sen) in enumerate(sims):
+                ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)["p"]
+                e = rel_l2(got[i], ref)
+                print(f"    {len(names[0][1])}d graph={use_graph} sim {i} rel-L2 {e:.3e}", flush=True)
+                assert e <= 1e-10, (i, use_graph, e)
+
+
+def test_batch_flags_rejected():
+    """Unknown kw_batch_create flag bits, and a grid cap without persistent grids, are KW_ERR_INVALID."""
+    import ctypes as C
+    from kwave_amd import _lib
+    sims, kw = _sims([("psrc", (32, 32))], nt=4)
+    d, k, _ = K.build_desc(*copy.deepcopy(sims[0]), data_cast=np.float64, **kw)
+    arr = (_lib.SimDesc * 1)(d)
+    lib = _lib.load()
+    for flags in (4, 1 | (1 << 8), -1):
+        h = C.c_void_p()
+        with pytest.raises(_lib.KWaveHIPError):
+            _lib.check(lib.kw_batch_create(_lib.context(0), arr, 1, flags, C.byref(h)))
+    del k
