@@ -89,6 +89,10 @@ template <typename T>
 hipError_t launch_c2r_batch(int n, unsigned nwg, dim3 b, size_t sh, hipStream_t s, const C2RArgs<T>* args,
                             const int* wg_sim, const int* wg_first, int cus = 0, int cap = 0);
 
+// Workgroups of `kern` (block threads, sh bytes of LDS) resident per CU of
+// the current device, queried once per (device, kernel, block, LDS) and kept.
+hipError_t occupancy_per_cu(const void* kern, int block, size_t sh, int* per_cu);
+
 // Launch of one batched kernel over nwg tiles (grid as described above).
 template <typename... P, typename... A>
 hipError_t launch_tiles(void (*kern)(P...), unsigned nwg, dim3 b, size_t sh, hipStream_t s, int cus, int cap,
@@ -96,8 +100,7 @@ hipError_t launch_tiles(void (*kern)(P...), unsigned nwg, dim3 b, size_t sh, hip
   unsigned g = nwg;
   if (cus > 0) {
     int per_cu = 0;
-    const hipError_t e =
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), (int)b.x, sh);
+    const hipError_t e = occupancy_per_cu(reinterpret_cast<const void*>(kern), (int)b.x, sh, &per_cu);
     if (e != hipSuccess) return e;
     unsigned fit = (unsigned)(per_cu > 0 ? per_cu : 1) * (unsigned)cus;
     if (cap > 0 && (unsigned)cap < fit) fit = (unsigned)cap;

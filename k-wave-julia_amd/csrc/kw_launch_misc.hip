@@ -1,7 +1,33 @@
 // Plan queries and the small per-step kernels (recording, injection, init).
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "kw_launch.hpp"
 
 namespace kw {
+
+hipError_t occupancy_per_cu(const void* kern, int block, size_t sh, int* per_cu) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void*, int, size_t>, int> known;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const auto key = std::make_tuple(dev, kern, block, sh);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = known.find(key);
+    if (it != known.end()) {
+      *per_cu = it->second;
+      return hipSuccess;
+    }
+  }
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, kern, block, sh);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(mu);
+  known[key] = *per_cu;
+  return hipSuccess;
+}
 
 bool size_supported(long long n) {
   switch (n) {
