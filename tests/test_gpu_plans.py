@@ -107,3 +107,54 @@ def test_autotune_plan_trials_then_parity():
     ref = O.kspace_first_order(copy.deepcopy(kg), med, src, sen, **kw)
     for k in out.keys():
         assert rel_l2(out[k], ref[k]) <= 1e-10, k
+
+
+@pytest.mark.parametrize("plan", [0, 1, 2])
+def test_forced_plan_family_axisymmetric_fp64(plan):
+    """The axisymmetric solver's radial pass (1/r term) under each plan family."""
+    print(f"\n[case] plan family {plan} axisymmetric fp64 256x256", flush=True)
+    kg, med, src, sen, kw = scenario("hetc_hetr_absorb_psrc", (256, 256), seed=21, nt=24)
+    try:
+        _force(plan, (256,))
+        out = K.kspace_first_order_as(copy.deepcopy(kg), med, src, sen, data_cast=np.float64, **kw)
+    finally:
+        tuning.clear_table()
+    ref = O.kspace_first_order_as(copy.deepcopy(kg), med, src, sen, **kw)
+    for k in out.keys():
+        e = rel_l2(out[k], ref[k])
+        print(f"    {k:8s} rel-L2 {e:.3e}", flush=True)
+        assert e <= 1e-10, (k, e)
+
+
+@pytest.mark.parametrize("plan", [0, 1, 2])
+def test_forced_plan_family_stacked_firings_fp64(plan):
+    """Stacked firings (one launch per pass over all firings) under each plan family equal
+    the default-family single runs."""
+    print(f"\n[case] plan family {plan} stacked firings fp64 256x256", flush=True)
+    kg = K.KWaveGrid(256, 1e-4, 256, 1e-4)
+    med = K.KWaveMedium(sound_speed=1500.0 + 40.0 * np.random.default_rng(3).standard_normal((256, 256)),
+                        density=1000.0, alpha_coeff=0.75, alpha_power=1.5, alpha_mode="stokes")
+    K.make_time(kg, 1700.0)
+    kg.Nt = 40
+    kg.t_array = np.arange(kg.Nt) * kg.dt
+    srcs, sens = [], []
+    for i in range(3):
+        m = np.zeros((256, 256), dtype=bool)
+        m[100 + 3 * i, 60 + 30 * i] = True
+        sig = 1e5 * K.tone_burst(1 / kg.dt, 1.5e6, 2)
+        srcs.append(K.KWaveSource(p_mask=m, p=sig[None, :]))
+        sm = np.zeros((256, 256), dtype=bool)
+        sm[110, 20:230 - 10 * i] = True   # within the distance the wave covers in 40 steps
+        sens.append(K.KWaveSensor(mask=sm, record=["p", "p_max"]))
+    kw = dict(pml_size=12, pml_alpha=2.0, data_cast=np.float64)
+    refs = [K.kspace_first_order(copy.deepcopy(kg), med, s, r, smooth_p0=False, **kw) for s, r in zip(srcs, sens)]
+    try:
+        _force(plan, (256,))
+        outs = K.kspace_first_order_firings(copy.deepcopy(kg), med, srcs, sens, **kw)
+    finally:
+        tuning.clear_table()
+    for i in range(3):
+        for f in ("p", "p_max"):
+            e = rel_l2(outs[i][f], refs[i][f])
+            print(f"    firing {i} {f:6s} rel-L2 {e:.3e}", flush=True)
+            assert e <= 1e-10, (i, f, e)
