@@ -25,7 +25,8 @@ module KWaveHIP
 
 using Libdl
 
-export kspace_first_order, kspace_first_order_as, gradient_c0, misfit, SimulationOutput,
+export kspace_first_order, kspace_first_order_as, kspace_first_order_batch, gradient_c0, misfit, SimulationOutput,
+       autotune!, tune_table, tune_load!, tune_clear!,
        KWaveGrid, KWaveGrid1D, KWaveGrid2D, KWaveGrid3D, KWaveMedium, KWaveSource, KWaveSensor,
        SourceMode, Dirichlet, Additive, AdditiveNoCorrection, make_time!
 
@@ -524,6 +525,107 @@ function gradient_c0(kgrid, medium, source, sensor, p_obs; checkpoint_every::Int
     end
     return total, grad
 end
+
+# ----------------------------------------------------------------------------
+# Batches of independent simulations (kw_batch_*) and the launch-shape tuner
+# ----------------------------------------------------------------------------
+const KW_BATCH_GRAPH = Int32(1)
+const KW_BATCH_PERSISTENT = Int32(2)
+
+"""
+    kspace_first_order_batch(sims; data_cast=Float32, use_graph=true, persistent=false,
+                             device=0, kwargs...) -> Vector{SimulationOutput}
+
+Runs a vector of `(kgrid, medium, source, sensor)` tuples as one batch: every
+pass of the time step is one launch per line length over all simulations and,
+with `use_graph`, one step is one hipGraph replay (`kw_batch_create`);
+`persistent` selects the persistent-grid launches.  Keyword arguments are
+those of `kspace_first_order` and apply to every simulation; all share
+dimensionality and Nt; time reversal is not batched.
+"""
+function kspace_first_order_batch(sims::AbstractVector; data_cast::Type=Float32, use_graph::Bool=true,
+                                  persistent::Bool=false, device::Integer=0, output_type=SimulationOutput,
+                                  kwargs...)
+    prepared = map(sims) do (kgrid, medium, source, sensor)
+        desc, keep, info = fill_desc(kgrid, medium, source, sensor; data_cast=data_cast, kwargs...)
+        info[:tr] && error("time reversal is not batched")
+        (desc, keep, info, sensor)
+    end
+    descs = [x[1] for x in prepared]
+    flags = (use_graph ? KW_BATCH_GRAPH : Int32(0)) | (persistent ? KW_BATCH_PERSISTENT : Int32(0))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve descs prepared begin
+        check(ccall(sym(:kw_batch_create), Int32, (Ptr{Cvoid}, Ptr{KwSimDesc}, Int32, Int32, Ref{Ptr{Cvoid}}),
+                    context(device), descs, Int32(length(descs)), flags, h))
+    end
+    outs = Any[]
+    try
+        Nt = prepared[1][3][:Nt]
+        check(ccall(sym(:kw_batch_run), Int32, (Ptr{Cvoid}, Int64), h[], Nt))
+        check(ccall(sym(:kw_batch_synchronize), Int32, (Ptr{Cvoid},), h[]))
+        for (i, (_, _, info, sensor)) in enumerate(prepared)
+            data = Dict{Symbol, AbstractArray}()
+            if sensor.mask !== nothing
+                for r in sensor.record
+                    if r in (:ux, :uy, :uz) && findfirst(==(r), (:ux, :uy, :uz)) > length(info[:shape])
+                        data[r] = zeros(Float64, info[:n_sensor], Nt)
+                        continue
+                    end
+                    rows, cols = Ref{Int64}(0), Ref{Int64}(0)
+                    check(ccall(sym(:kw_batch_sensor_shape), Int32,
+                                (Ptr{Cvoid}, Int32, UInt32, Ref{Int64}, Ref{Int64}), h[], i - 1, REC[r], rows, cols))
+                    v = Vector{Float64}(undef, rows[] * cols[])
+                    check(ccall(sym(:kw_batch_get_sensor), Int32, (Ptr{Cvoid}, Int32, UInt32, Ptr{Float64}),
+                                h[], i - 1, REC[r], v))
+                    data[r] = cols[] == 1 ? v : reshape(v, Int(rows[]), Int(cols[]))
+                end
+            end
+            push!(outs, output_type(data))
+        end
+    finally
+        ccall(sym(:kw_batch_destroy), Int32, (Ptr{Cvoid},), h[])
+    end
+    return outs
+end
+
+"""
+    autotune!(kgrid, medium, source, sensor; steps=10, data_cast=Float32, device=0, kwargs...) -> String
+
+Times the launch shapes of this simulation's passes (lines per workgroup and,
+at line lengths 256 and 512, the plan family) with `kw_autotune` and keeps the
+fastest in the library's process-wide table, which later simulations read: the
+role FFTW.MEASURE plays in `create_fft_plans` (fft_utils.jl:44-62).  Returns
+the table as text (`tune_table`).
+"""
+function autotune!(kgrid, medium, source, sensor; steps::Integer=10, data_cast::Type=Float32,
+                   device::Integer=0, kwargs...)
+    desc, keep, _ = fill_desc(kgrid, medium, source, sensor; data_cast=data_cast, kwargs...)
+    GC.@preserve keep begin
+        check(ccall(sym(:kw_autotune), Int32, (Ptr{Cvoid}, Ref{KwSimDesc}, Int32, Int32),
+                    context(device), desc, Int32(1), Int32(steps)))
+    end
+    return tune_table()
+end
+
+"""
+    tune_table() -> String
+
+The tuning table as text, one entry per line: precision bytes, pass class,
+line length, lines per workgroup and, where not the default, the plan family.
+"""
+function tune_table()
+    n = Ref{Int64}(0)
+    check(ccall(sym(:kw_tune_export), Int32, (Ptr{UInt8}, Int64, Ref{Int64}), C_NULL, 0, n))
+    buf = Vector{UInt8}(undef, n[] + 1)
+    check(ccall(sym(:kw_tune_export), Int32, (Ptr{UInt8}, Int64, Ref{Int64}), buf, length(buf), n))
+    return String(buf[1:n[]])
+end
+
+"""Merges a table in `tune_table`'s text form (e.g. read from a cache file)."""
+tune_load!(text::AbstractString) = check(ccall(sym(:kw_tune_import), Int32, (Cstring,), text))
+
+"""Empties the tuning table (defaults everywhere)."""
+tune_clear!() = check(ccall(sym(:kw_tune_clear), Int32, ()))
 
 # medium with its sound speed replaced (same struct, same other fields)
 _with_sound_speed(m, c0) = typeof(m)(Float64.(c0), m.density, m.alpha_coeff, m.alpha_power, m.alpha_mode, m.BonA)
