@@ -532,6 +532,18 @@ end
 const KW_BATCH_GRAPH = Int32(1)
 const KW_BATCH_PERSISTENT = Int32(2)
 
+# keywords of kspace_first_order: those fill_desc takes, and those the
+# reference accepts without using them in the loop (dropped here as there)
+const _DESC_KEYS = (:pml_size, :pml_alpha, :smooth_p0, :medium_storage, :axisymmetric)
+const _UNUSED_KEYS = (:pml_inside, :smooth_c0, :smooth_rho0, :plot_sim, :plot_layout, :plot_scale, :record_movie,
+                      :show_progress)
+function _desc_kwargs(kwargs)
+    for k in keys(kwargs)
+        k in _DESC_KEYS || k in _UNUSED_KEYS || error("unsupported keyword argument $k")
+    end
+    return (; (k => v for (k, v) in pairs(kwargs) if k in _DESC_KEYS)...)
+end
+
 """
     kspace_first_order_batch(sims; data_cast=Float32, use_graph=true, persistent=false,
                              device=0, kwargs...) -> Vector{SimulationOutput}
@@ -547,7 +559,7 @@ function kspace_first_order_batch(sims::AbstractVector; data_cast::Type=Float32,
                                   persistent::Bool=false, device::Integer=0, output_type=SimulationOutput,
                                   kwargs...)
     prepared = map(sims) do (kgrid, medium, source, sensor)
-        desc, keep, info = fill_desc(kgrid, medium, source, sensor; data_cast=data_cast, kwargs...)
+        desc, keep, info = fill_desc(kgrid, medium, source, sensor; data_cast=data_cast, _desc_kwargs(kwargs)...)
         info[:tr] && error("time reversal is not batched")
         (desc, keep, info, sensor)
     end
@@ -599,7 +611,7 @@ the table as text (`tune_table`).
 """
 function autotune!(kgrid, medium, source, sensor; steps::Integer=10, data_cast::Type=Float32,
                    device::Integer=0, kwargs...)
-    desc, keep, _ = fill_desc(kgrid, medium, source, sensor; data_cast=data_cast, kwargs...)
+    desc, keep, _ = fill_desc(kgrid, medium, source, sensor; data_cast=data_cast, _desc_kwargs(kwargs)...)
     GC.@preserve keep begin
         check(ccall(sym(:kw_autotune), Int32, (Ptr{Cvoid}, Ref{KwSimDesc}, Int32, Int32),
                     context(device), desc, Int32(1), Int32(steps)))
