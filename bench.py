@@ -586,7 +586,13 @@ def main():
     fp64_check = None
     if not a.no_fp64:
         fp64_check = accuracy_check(a.N, a.accuracy_nt)
-    forms = None if a.no_forms else other_forms()
+    forms = None
+    if not a.no_forms:
+        try:
+            forms = other_forms()
+        except Exception as e:   # the headline line stands on its own; say what failed
+            print(f"[bench] forms failed: {e!r}", file=sys.stderr, flush=True)
+            forms = {"error": repr(e)}
 
     N = a.N
     vox = N ** 3
