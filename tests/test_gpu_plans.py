@@ -30,7 +30,7 @@ def _force(plan, lengths, lpw=4, prec=(4, 8)):
 
 
 FWD = [("hetc_absorb_psrc", (256, 256), 24), ("hetc_hetr_nonlin_psrc_urec", (512, 256), 16),
-       ("hetc_absorb_p0", (16, 256, 256), 10)]
+       ("hetc_absorb_p0", (16, 256, 256), 10), ("stokes_p0_psrc", (512,), 60)]
 
 
 @pytest.mark.parametrize("plan", [0, 1, 2])
