@@ -199,3 +199,22 @@ def test_time_reversal_data_rows_checked():
     sen = K.KWaveSensor(mask=m, time_reversal_boundary_data=np.zeros((2, 40)))
     d, keep, info = K.build_desc(kg, med, K.KWaveSource(), sen)
     assert info["tr"] and d.Nt == 40 and d.n_sensor == 2
+
+
+def test_tuning_table_file_round_trip_keeps_plan_families(tmp_path):
+    """save_table / load_table carry the optional plan-family column unchanged."""
+    from kwave_amd import _lib, tuning
+    lib = _lib.load()
+    tuning.clear_table()
+    try:
+        _lib.check(lib.kw_tune_import(b"4 0 256 2 2\n4 3 512 8\n8 5 256 4 0\n"))
+        before_t, before_p = tuning.table(), tuning.plans()
+        path = str(tmp_path / "kw_tune.txt")
+        tuning.save_table(path)
+        tuning.clear_table()
+        assert tuning.table() == {} and tuning.plans() == {}
+        tuning.load_table(path)
+        assert tuning.table() == before_t and tuning.plans() == before_p
+        assert before_p == {(4, "dir_x", 256): 2, (8, "c2r_x", 256): 0}
+    finally:
+        tuning.clear_table()
