@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY.  This module is the oracle: it is imported by
 ``tests/``, by ``__graft_entry__.smoke()`` and by the ``cpu_baseline`` leg of
-``bench.py`` -- as the checker and the CPU baseline, never as the product.
+``bench.py`` -- as the parity reference and the CPU baseline, never as the product.
 The product path (``kwave_amd`` + the HIP library) must not import it.
 
 Every function names the reference file:line it restates (paths relative to
